@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Benchmark of the Doppler-moment reconstruction path on one MI355X.
+
+Workload (the headline configuration): a 4096-frame synthetic uint16 (12-bit) stream of
+1024x1024 frames (disk radius 400 px, f_s = 37 kHz), resident in HBM; Fresnel propagation
+(lambda 852 nm, pitch 12 um, z 0.45 m); sliding time window T = 64, stride 16 -> 253
+windows per step; band 6-15 kHz (bins [11, 26), 15 planes); M0/M1/M2 with fftshift.
+One step = the fused path over all 253 windows.  value = moment frames (windows) per
+second over the timed steps.
+
+Also reported on the same line: the mixed-precision MFMA time-DFT variants (throughput and
+relative L2 of every moment image against the FP64 oracle), per-window GPU latency
+percentiles, the roofline of the dominant kernel (the time DFT), and the CPU baseline
+(the oracle's FP32 torch-CPU/MKL restatement on a bounded sample of windows).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+H = W = 1024
+T, S = 64, 16
+FS = 37e3
+N_FRAMES = 4096
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_stream(n_frames, hfx, torch):
+    d = hfx.synth_desc(H, W, disk_radius_px=400.0, fs_hz=FS, seed=2024)
+    ring = torch.empty((n_frames, H, W), dtype=torch.int16, device="cuda")
+    phase = torch.zeros(H * W, dtype=torch.float64, device="cuda")
+    chunk = 256
+    for f0 in range(0, n_frames, chunk):
+        n = min(chunk, n_frames - f0)
+        hfx.synth_frames(d, f0, n, out=ring[f0:f0 + n], phase_state=phase)
+    torch.cuda.synchronize()
+    return ring
+
+
+def time_batch(plan, ring, n_windows, out, steps, warmup, torch):
+    for _ in range(warmup):
+        plan.moments_batch(ring, 0, n_windows, out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.moments_batch(ring, 0, n_windows, out)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--frames", type=int, default=N_FRAMES)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--kernel-loop", type=int, default=200,
+                    help="launches of the dominant kernel timed with HIP events")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note "
+                                   "only (see DESIGN.md), so --gpus must be 1"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from holoflow_amd import hfx
+
+    torch.cuda.set_device(0)
+    kb, ke = 11, 26
+    nb = ke - kb
+    npix = H * W
+    n_windows = (args.frames - T) // S + 1
+    log(f"generating {args.frames} frames of {H}x{W} on the GPU")
+    ring = make_stream(args.frames, hfx, torch)
+
+    def cfg(precision):
+        return hfx.MomentConfig(H, W, T, S, kb, ke, "fresnel", 852e-9, 12e-6, 12e-6, 0.45, fs_hz=FS,
+                                weight_fs_hz=FS, precision=precision, fftshift=True)
+
+    plan = hfx.MomentPlan(cfg("fp32"))
+    out = plan.new_output(n_windows)
+    log(f"timing fp32: {args.warmup} warmup + {args.steps} steps of {n_windows} windows")
+    dt = time_batch(plan, ring, n_windows, out, args.steps, args.warmup, torch)
+    value = n_windows * args.steps / dt
+    ms_per_step = dt / args.steps * 1e3
+    log(f"fp32: {value:.1f} moment frames/s, {ms_per_step:.2f} ms/step")
+
+    # ---- dominant kernel (time DFT) timed with HIP events on the launch stream
+    planes = torch.empty((plan.n_planes, H, W), dtype=torch.complex64, device="cuda")
+    from holoflow_amd._lib import check, lib
+    import ctypes as C
+
+    stream = torch.cuda.current_stream()
+    sp = C.c_void_p(stream.cuda_stream)
+
+    def launch_dft(first):
+        check(lib().hfx_plan_time_dft(plan._h, C.c_void_p(ring.data_ptr()), ring.shape[0], first,
+                                      C.c_void_p(planes.data_ptr()), sp))
+
+    for i in range(5):
+        launch_dft(i * S)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for i in range(args.kernel_loop):
+        launch_dft((i % n_windows) * S)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    dft_ms = e0.elapsed_time(e1) / args.kernel_loop
+    dft_bytes = T * npix * 2 + nb * npix * 8
+    dft_gbs = dft_bytes / (dft_ms * 1e-3) / 1e9
+    log(f"time-DFT kernel: {dft_ms * 1e3:.1f} us/launch, {dft_gbs:.0f} GB/s")
+
+    path_bytes = T * npix * 2 + 2 * nb * npix * 8 + 3 * npix * 4
+    path_gbs = path_bytes * value / 1e9
+
+    # ---- per-window GPU latency (events around each window's kernels)
+    lat = []
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(n_windows)]
+    one = plan.new_output()
+    for w in range(n_windows):
+        evs[w][0].record(stream)
+        plan.moments(ring, w * S, one)
+        evs[w][1].record(stream)
+    torch.cuda.synchronize()
+    lat = np.array([a.elapsed_time(b) for a, b in evs])
+    window_period_ms = S / FS * 1e3
+
+    # ---- mixed-precision MFMA time DFT: throughput + error vs FP64 oracle (window 0)
+    mixed = {}
+    from oracle import oracle as O
+
+    p_or = O.config_params("cfg1_1024_fresnel")
+    win0 = ring[:T].cpu().numpy().view(np.uint16)
+    ref64 = None
+    if not args.no_mixed:
+        log("FP64 oracle for window 0")
+        ref64 = O.doppler_moments(win0, p_or, "f64")
+        fp32_err = [O.rel_l2(plan.moments(ring, 0).cpu().numpy()[i], ref64[i]) for i in range(3)]
+        for prec in ("fp16_split", "fp16", "bf16"):
+            mp = hfx.MomentPlan(cfg(prec))
+            mdt = time_batch(mp, ring, n_windows, out, args.steps, 1, torch)
+            m = mp.moments(ring, 0).cpu().numpy()
+            mixed[prec] = {
+                "value": round(n_windows * args.steps / mdt, 2),
+                "speedup_vs_fp32": round(dt / mdt, 4),
+                "rel_l2_vs_fp64": {k: float(f"{O.rel_l2(m[i], ref64[i]):.3e}") for i, k in
+                                   enumerate(("M0", "M1", "M2"))},
+            }
+            log(f"{prec}: {mixed[prec]}")
+            mp.close()
+        mixed["fp32_rel_l2_vs_fp64"] = {k: float(f"{fp32_err[i]:.3e}") for i, k in enumerate(("M0", "M1", "M2"))}
+
+    # ---- CPU baseline: the oracle's FP32 restatement (torch-CPU, MKL FFTs), host cores
+    cpu = None
+    if not args.no_cpu:
+        threads = torch.get_num_threads()
+        max_w = 16
+        host = ring[:T + S * (max_w - 1)].cpu().numpy().view(np.uint16)
+        O.doppler_moments(host[:T], p_or, "f32")        # warm-up
+        t0 = time.perf_counter()
+        done = 0
+        while done < max_w and (done < 2 or time.perf_counter() - t0 < args.cpu_seconds):
+            O.doppler_moments(host[done * S:done * S + T], p_or, "f32")
+            done += 1
+        cdt = time.perf_counter() - t0
+        cpu = {"value": round(done / cdt, 4), "unit": "moment frames/s", "cores": threads,
+               "kind": "port",
+               "sample": f"{done} consecutive windows (T=64, stride 16) of the same 1024x1024 stream, "
+                         f"FP32 torch-CPU (MKL) restatement, {cdt:.1f} s"}
+        log(f"cpu baseline: {cpu}")
+
+    line = {
+        "metric": "moment frames/s @1024^2 win64 (FP32 vs mixed MFMA); p99 latency; HBM GB/s frac",
+        "value": round(value, 2),
+        "unit": "moment frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Philox speckle interferograms generated on the GPU)",
+        "config": {"workload": "1024x1024 u16 12-bit, 4096-frame stream, Fresnel z=0.45 m, "
+                               "T=64 stride 16 (253 windows/step), bins [11,26), M0/M1/M2 + fftshift",
+                   "frames": args.frames, "windows_per_step": n_windows, "precision": "fp32"},
+        "roofline": {"bound": "hbm", "achieved": round(dft_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(dft_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "time DFT (k_dft_f32)", "kernel_us": round(dft_ms * 1e3, 2),
+                     "algorithmic_bytes_per_launch": dft_bytes},
+        "path_roofline": {"achieved": round(path_gbs, 1), "frac": round(path_gbs / HBM_PEAK_GBS, 4),
+                          "algorithmic_bytes_per_window": path_bytes},
+        "latency_ms": {"window_gpu_p50": round(float(np.percentile(lat, 50)), 4),
+                       "window_gpu_p99": round(float(np.percentile(lat, 99)), 4),
+                       "window_period": round(window_period_ms, 4)},
+        "mixed": mixed,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,157 @@
+"""ctypes binding of libhfx.so (the C ABI declared in include/hfx.h).
+
+There is no fallback: if the shared library is missing or fails to load, every entry
+point raises.  Device buffers are passed as raw integer addresses (for example
+``torch.Tensor.data_ptr()``) and streams as raw hipStream_t handles.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhfx.so")
+
+HFX_OK = 0
+HFX_E_INVALID = 1
+HFX_E_UNSUPPORTED = 2
+HFX_E_HIP = 3
+HFX_E_NOMEM = 4
+
+PROP_FRESNEL = 0
+PROP_ANGULAR_SPECTRUM = 1
+
+PREC_FP32 = 0
+PREC_FP16 = 1
+PREC_BF16 = 2
+PREC_FP16_SPLIT = 3
+PRECISIONS = {"fp32": PREC_FP32, "fp16": PREC_FP16, "bf16": PREC_BF16, "fp16_split": PREC_FP16_SPLIT}
+
+FLAG_FFTSHIFT = 1
+FLAG_POWER2 = 2
+FLAG_BOTH_SIDEBANDS = 4
+
+DTYPE_U16 = 1
+DTYPE_F32 = 2
+DTYPE_CF32 = 3
+
+
+class HfxError(RuntimeError):
+    """Raised for HIP / unsupported failures reported by libhfx."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"hfx error {code}: {msg}")
+        self.code = code
+
+
+class HfxInvalidArgument(HfxError, ValueError):
+    """HFX_E_INVALID: the reference's std::invalid_argument."""
+
+
+class PlanDesc(C.Structure):
+    _fields_ = [
+        ("height", C.c_int32), ("width", C.c_int32),
+        ("window", C.c_int32), ("stride", C.c_int32),
+        ("fs_hz", C.c_double),
+        ("lambda_m", C.c_double), ("dx_m", C.c_double), ("dy_m", C.c_double), ("z_m", C.c_double),
+        ("bin_begin", C.c_int32), ("bin_end", C.c_int32),
+        ("prop", C.c_int32), ("precision", C.c_int32), ("flags", C.c_uint32),
+        ("pad_height", C.c_int32), ("pad_width", C.c_int32),
+        ("filter_enabled", C.c_int32),
+        ("r_inner", C.c_double), ("r_outer", C.c_double), ("s_inner", C.c_double), ("s_outer", C.c_double),
+        ("split_bin", C.c_int32),
+        ("weight_fs_hz", C.c_double),
+    ]
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [
+        ("out_height", C.c_int32), ("out_width", C.c_int32),
+        ("n_outputs", C.c_int32), ("n_planes", C.c_int32),
+        ("workspace_bytes", C.c_int64),
+    ]
+
+
+class AsmDesc(C.Structure):
+    _fields_ = [
+        ("batch", C.c_int64),
+        ("height", C.c_int32), ("width", C.c_int32),
+        ("lambda_m", C.c_double), ("dx_m", C.c_double), ("dy_m", C.c_double), ("z_m", C.c_double),
+        ("in_real", C.c_int32),
+        ("pad_height", C.c_int32), ("pad_width", C.c_int32),
+        ("filter_enabled", C.c_int32),
+        ("r_inner", C.c_double), ("r_outer", C.c_double), ("s_inner", C.c_double), ("s_outer", C.c_double),
+    ]
+
+
+class SynthDesc(C.Structure):
+    _fields_ = [
+        ("height", C.c_int32), ("width", C.c_int32),
+        ("disk_radius_px", C.c_double),
+        ("fs_hz", C.c_double),
+        ("ref_amp", C.c_double), ("obj_sigma", C.c_double), ("gain", C.c_double),
+        ("carrier_cyc_per_px", C.c_double), ("phase_noise_rad", C.c_double), ("fd_max_hz", C.c_double),
+        ("seed", C.c_uint32), ("stream_id", C.c_uint32),
+    ]
+
+
+_VP = C.c_void_p
+_P = C.POINTER
+# name -> (restype, argtypes); every symbol declared in include/hfx.h
+SIGNATURES = {
+    "hfx_last_error": (C.c_char_p, []),
+    "hfx_abi_version": (C.c_int32, []),
+    "hfx_plan_desc_init": (None, [_P(PlanDesc)]),
+    "hfx_plan_create": (C.c_int, [_P(PlanDesc), _VP, _P(_VP)]),
+    "hfx_plan_destroy": (C.c_int, [_VP]),
+    "hfx_plan_get_info": (C.c_int, [_VP, _P(PlanInfo)]),
+    "hfx_plan_update_z": (C.c_int, [_VP, C.c_double, _VP]),
+    "hfx_moments": (C.c_int, [_VP, _VP, C.c_int64, C.c_int64, _VP, _VP]),
+    "hfx_moments_batch": (C.c_int, [_VP, _VP, C.c_int64, C.c_int64, C.c_int32, C.c_int64, _VP, _VP]),
+    "hfx_plan_time_dft": (C.c_int, [_VP, _VP, C.c_int64, C.c_int64, _VP, _VP]),
+    "hfx_dft_axis": (C.c_int, [_VP, C.c_int32, C.c_int64, C.c_int32, C.c_int64, C.c_int32, C.c_int32, _VP, _VP]),
+    "hfx_fresnel_create": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,
+                                     C.c_double, C.c_int32, C.c_int32, C.c_int32, _VP, _P(_VP)]),
+    "hfx_fresnel_update_z": (C.c_int, [_VP, C.c_double, _VP]),
+    "hfx_fresnel_execute": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "hfx_fresnel_destroy": (C.c_int, [_VP]),
+    "hfx_asm_create": (C.c_int, [_P(AsmDesc), _VP, _P(_VP)]),
+    "hfx_asm_execute": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "hfx_asm_destroy": (C.c_int, [_VP]),
+    "hfx_weighted_mean": (C.c_int, [_VP, C.c_int32, C.c_int64, C.c_int64, _VP, C.c_int32, _VP, _VP]),
+    "hfx_fftshift": (C.c_int, [_VP, _VP, C.c_int32, C.c_int32, _P(C.c_int64), _P(C.c_int32), _VP]),
+    "hfx_convert_u16": (C.c_int, [_VP, _VP, C.c_int64, C.c_int32, _VP]),
+    "hfx_synth_desc_init": (None, [_P(SynthDesc), C.c_int32, C.c_int32]),
+    "hfx_synth_frames": (C.c_int, [_P(SynthDesc), _VP, C.c_int64, C.c_int64, _VP, _VP]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def lib() -> C.CDLL:
+    """Loads libhfx.so once; raises if it is missing (there is no fallback path)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(
+                    f"{LIB_PATH} is missing: build it with `python holoflow_amd/build.py` "
+                    "(hipcc, gfx950). The HIP path has no CPU fallback.")
+            L = C.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(L, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = L
+    return _lib
+
+
+def check(code: int) -> None:
+    if code == HFX_OK:
+        return
+    msg = lib().hfx_last_error().decode(errors="replace")
+    if code == HFX_E_INVALID:
+        raise HfxInvalidArgument(code, msg)
+    raise HfxError(code, msg)

@@ -1,0 +1,647 @@
+// C ABI of libhfx: plan construction (validation, device tables, workspaces) and the
+// launch sequences of the fused Doppler-moment path and the per-node kinds.
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/hfx.h"
+#include "hfx_kernels.hh"
+
+namespace {
+
+thread_local std::string g_err;
+
+hfx_status fail(hfx_status code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HFX_TRY_HIP(expr)                                                              \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(HFX_E_HIP, std::string(#expr " failed: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+inline hipStream_t S(hfx_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Device allocation that records itself for release.
+struct DevArena {
+  std::vector<void *> ptrs;
+  size_t bytes = 0;
+  template <typename T> hfx_status alloc(T **p, size_t count) {
+    void *v = nullptr;
+    size_t b = count * sizeof(T);
+    if (b == 0) b = 16;
+    hipError_t e = hipMalloc(&v, b);
+    if (e != hipSuccess) return fail(HFX_E_NOMEM, "hipMalloc of " + std::to_string(b) + " bytes failed");
+    ptrs.push_back(v);
+    bytes += b;
+    *p = static_cast<T *>(v);
+    return HFX_OK;
+  }
+  ~DevArena() {
+    for (void *p : ptrs) (void)hipFree(p);
+  }
+};
+
+#define HFX_TRY(expr)              \
+  do {                             \
+    hfx_status s_ = (expr);        \
+    if (s_ != HFX_OK) return s_;   \
+  } while (0)
+
+// M1/M2 weights exactly as the reference builds them: Arange in double, cast to F32
+// (arange.cu:61-68), negative band at f - fs (graph_builder.cc:578-601), squares in F32
+// (Multiply of two F32 tensors, graph_builder.cc:626).
+void build_weights(int kb, int ke, int T, double fs_w, bool both, std::vector<float> &f,
+                   std::vector<float> &f2) {
+  const double df = fs_w / (double)T;
+  const double f0 = kb * df;
+  const int nb = ke - kb;
+  f.clear();
+  for (int i = 0; i < nb; ++i) f.push_back((float)(f0 + df * (double)i));
+  if (both)
+    for (int i = 0; i < nb; ++i) f.push_back((float)((f0 - fs_w) + df * (double)i));
+  f2.resize(f.size());
+  for (size_t i = 0; i < f.size(); ++i) f2[i] = f[i] * f[i];
+}
+
+}  // namespace
+
+struct hfx_plan {
+  hfx_plan_desc d{};
+  int H = 0, W = 0, OH = 0, OW = 0;
+  long long npix = 0, onpix = 0;
+  int nb = 0, nbc = 0, n_planes = 0, n_out = 3, blk_c = 1, wm_rows = 0;
+  DevArena arena;
+  float2 *planes_nat = nullptr, *planes_blk = nullptr;
+  float2 *tw_w = nullptr, *tw_h = nullptr;
+  float *wm32 = nullptr;
+  __half *wm16hi = nullptr, *wm16lo = nullptr;
+  __hip_bfloat16 *wmbf = nullptr;
+  float2 *lx = nullptr, *ly = nullptr;
+  float2 *tf = nullptr;
+  float *wf = nullptr, *wf2 = nullptr;
+};
+
+struct hfx_fresnel {
+  long long batch = 0;
+  int H = 0, W = 0, in_real = 0, output_magnitude = 0, skip_phase_shift = 0;
+  double lambda = 0, dx = 0, dy = 0, z = 0;
+  int blk_c = 1;
+  DevArena arena;
+  float2 *tw_w = nullptr, *tw_h = nullptr, *lx = nullptr, *ly = nullptr, *ws = nullptr;
+};
+
+struct hfx_asm {
+  hfx_asm_desc d{};
+  int OH = 0, OW = 0, blk_c = 1;
+  DevArena arena;
+  float2 *tw_w = nullptr, *tw_h = nullptr, *tf = nullptr, *ws = nullptr;
+};
+
+extern "C" {
+
+const char *hfx_last_error(void) { return g_err.c_str(); }
+int32_t hfx_abi_version(void) { return 1; }
+
+void hfx_plan_desc_init(hfx_plan_desc *d) {
+  std::memset(d, 0, sizeof(*d));
+  d->height = 512;
+  d->width = 512;
+  d->window = 32;
+  d->stride = 32;
+  d->fs_hz = 37e3;
+  d->lambda_m = 852e-9;
+  d->dx_m = 12e-6;
+  d->dy_m = 12e-6;
+  d->z_m = 0.3;
+  d->bin_begin = 6;
+  d->bin_end = 13;
+  d->prop = HFX_PROP_FRESNEL;
+  d->precision = HFX_PREC_FP32;
+  d->flags = 0;
+  d->weight_fs_hz = 37e3;
+}
+
+static hfx_status gen_prop_tables(hfx_plan *p, hipStream_t st) {
+  const hfx_plan_desc &d = p->d;
+  if (d.prop == HFX_PROP_FRESNEL) {
+    hfx::launch_fresnel_chirp(p->lx, p->ly, p->H, p->W, d.lambda_m, d.z_m, d.dx_m, d.dy_m, st);
+  } else {
+    hfx::launch_asm_transfer(p->tf, p->OH, p->OW, d.lambda_m, d.z_m, d.dx_m, d.dy_m,
+                             d.filter_enabled, d.r_inner, d.r_outer, d.s_inner, d.s_outer, p->blk_c,
+                             st);
+  }
+  HFX_TRY_HIP(hipGetLastError());
+  return HFX_OK;
+}
+
+hfx_status hfx_plan_create(const hfx_plan_desc *dp, hfx_stream_t stream, hfx_plan **out) {
+  if (!dp || !out) return fail(HFX_E_INVALID, "null argument");
+  *out = nullptr;
+  const hfx_plan_desc &d = *dp;
+  if (d.height <= 0 || d.width <= 0) return fail(HFX_E_INVALID, "frame size must be positive");
+  if (d.window <= 0 || d.stride <= 0) return fail(HFX_E_INVALID, "window and stride must be positive");
+  if (d.window % 4 != 0) return fail(HFX_E_UNSUPPORTED, "window must be a multiple of 4");
+  if (d.precision != HFX_PREC_FP32 && d.window % 32 != 0)
+    return fail(HFX_E_UNSUPPORTED, "16-bit MFMA time DFT needs a window multiple of 32");
+  if (d.precision < 0 || d.precision > 3) return fail(HFX_E_INVALID, "unknown precision");
+  if (d.width % 4 != 0) return fail(HFX_E_UNSUPPORTED, "frame width must be a multiple of 4");
+  if (!(d.lambda_m > 0)) return fail(HFX_E_INVALID, "wavelength must be positive");
+  if (!(d.dx_m > 0) || !(d.dy_m > 0)) return fail(HFX_E_INVALID, "pixel pitch must be positive");
+  if (d.dx_m != d.dy_m) return fail(HFX_E_INVALID, "dx must equal dy");
+  const bool both = (d.flags & HFX_FLAG_BOTH_SIDEBANDS) != 0;
+  const int half = d.window / 2 + 1;
+  if (d.bin_begin < 0 || d.bin_end <= d.bin_begin)
+    return fail(HFX_E_INVALID, "band must satisfy 0 <= bin_begin < bin_end");
+  if (!both && d.bin_end > half)
+    return fail(HFX_E_INVALID, "bin_end must be <= window/2 + 1 for the one-sided DFT");
+  if (both && d.bin_end > d.window) return fail(HFX_E_INVALID, "bin_end must be <= window");
+  if (d.prop != HFX_PROP_FRESNEL && d.prop != HFX_PROP_ANGULAR_SPECTRUM)
+    return fail(HFX_E_INVALID, "unknown propagation");
+  if (d.prop == HFX_PROP_FRESNEL && !(d.z_m != 0)) return fail(HFX_E_INVALID, "Fresnel z must be non-zero");
+  if (d.prop == HFX_PROP_FRESNEL && (d.pad_height || d.pad_width || d.filter_enabled))
+    return fail(HFX_E_INVALID, "padding and filter apply to the angular spectrum only");
+  const int nb = d.bin_end - d.bin_begin;
+  if (d.split_bin != 0 && (d.split_bin <= d.bin_begin || d.split_bin >= d.bin_end || both))
+    return fail(HFX_E_INVALID, "split_bin must lie strictly inside the one-sided band");
+
+  auto *p = new (std::nothrow) hfx_plan();
+  if (!p) return fail(HFX_E_NOMEM, "host allocation failed");
+  std::unique_ptr<hfx_plan> guard(p);
+  p->d = d;
+  p->H = d.height;
+  p->W = d.width;
+  p->OH = d.pad_height > 0 ? d.pad_height : d.height;
+  p->OW = d.pad_width > 0 ? d.pad_width : d.width;
+  if (p->OH < p->H || p->OW < p->W || (p->OH - p->H) % 2 || (p->OW - p->W) % 2)
+    return fail(HFX_E_INVALID, "padding must be >= the frame size with even margins");
+  p->npix = (long long)p->H * p->W;
+  p->onpix = (long long)p->OH * p->OW;
+  p->nb = nb;
+  p->nbc = both ? nb + 1 : nb;
+  if (p->nbc > 64) return fail(HFX_E_UNSUPPORTED, "at most 64 band bins per plan");
+  p->n_planes = both ? 2 * nb : nb;
+  p->n_out = d.split_bin > 0 ? 5 : 3;
+  p->blk_c = hfx::pick_blk_c(p->OW, p->OH);
+  if (!hfx::row_pass_supported(p->OW) || !hfx::col_pass_supported(p->OH, p->blk_c))
+    return fail(HFX_E_UNSUPPORTED, "propagation size must be one of the compiled 2^a*3^b lengths "
+                                   "(2..2048)");
+  p->wm_rows = ((2 * p->nbc + 15) / 16) * 16;
+
+  hipStream_t st = S(stream);
+  DevArena &A = p->arena;
+  HFX_TRY(A.alloc(&p->planes_nat, (size_t)p->n_planes * p->npix));
+  HFX_TRY(A.alloc(&p->planes_blk, (size_t)p->n_planes * p->onpix));
+  HFX_TRY(A.alloc(&p->tw_w, p->OW));
+  HFX_TRY(A.alloc(&p->tw_h, p->OH));
+  HFX_TRY(A.alloc(&p->wm32, (size_t)p->wm_rows * d.window));
+  HFX_TRY(A.alloc(&p->wm16hi, (size_t)p->wm_rows * d.window));
+  HFX_TRY(A.alloc(&p->wm16lo, (size_t)p->wm_rows * d.window));
+  HFX_TRY(A.alloc(&p->wmbf, (size_t)p->wm_rows * d.window));
+  if (d.prop == HFX_PROP_FRESNEL) {
+    HFX_TRY(A.alloc(&p->lx, p->W));
+    HFX_TRY(A.alloc(&p->ly, p->H));
+  } else {
+    HFX_TRY(A.alloc(&p->tf, (size_t)p->onpix));
+  }
+  HFX_TRY(A.alloc(&p->wf, p->n_planes));
+  HFX_TRY(A.alloc(&p->wf2, p->n_planes));
+
+  hfx::launch_fft_twiddles(p->tw_w, p->OW, st);
+  hfx::launch_fft_twiddles(p->tw_h, p->OH, st);
+  hfx::launch_dft_matrix(p->wm32, p->wm16hi, p->wm16lo, p->wmbf, p->wm_rows, d.window,
+                         d.bin_begin, p->nbc, st);
+  HFX_TRY(gen_prop_tables(p, st));
+  std::vector<float> f, f2;
+  double fs_w = d.weight_fs_hz > 0 ? d.weight_fs_hz : d.fs_hz;
+  build_weights(d.bin_begin, d.bin_end, d.window, fs_w, both, f, f2);
+  HFX_TRY_HIP(hipMemcpyAsync(p->wf, f.data(), f.size() * 4, hipMemcpyHostToDevice, st));
+  HFX_TRY_HIP(hipMemcpyAsync(p->wf2, f2.data(), f2.size() * 4, hipMemcpyHostToDevice, st));
+  HFX_TRY_HIP(hipStreamSynchronize(st));
+  *out = guard.release();
+  return HFX_OK;
+}
+
+hfx_status hfx_plan_destroy(hfx_plan *p) {
+  delete p;
+  return HFX_OK;
+}
+
+hfx_status hfx_plan_get_info(const hfx_plan *p, hfx_plan_info *info) {
+  if (!p || !info) return fail(HFX_E_INVALID, "null argument");
+  info->out_height = p->OH;
+  info->out_width = p->OW;
+  info->n_outputs = p->n_out;
+  info->n_planes = p->n_planes;
+  info->workspace_bytes = (int64_t)p->arena.bytes;
+  return HFX_OK;
+}
+
+hfx_status hfx_plan_update_z(hfx_plan *p, double z_m, hfx_stream_t stream) {
+  if (!p) return fail(HFX_E_INVALID, "null plan");
+  if (p->d.prop == HFX_PROP_FRESNEL && !(z_m != 0)) return fail(HFX_E_INVALID, "Fresnel z must be non-zero");
+  p->d.z_m = z_m;
+  HFX_TRY(gen_prop_tables(p, S(stream)));
+  return HFX_OK;
+}
+
+static hfx_status check_ring(const hfx_plan *p, const uint16_t *ring, int64_t ring_frames,
+                             int64_t first) {
+  if (!p || !ring) return fail(HFX_E_INVALID, "null argument");
+  if (ring_frames < p->d.window) return fail(HFX_E_INVALID, "ring holds fewer frames than the window");
+  if (first < 0 || first >= ring_frames) return fail(HFX_E_INVALID, "first_frame outside the ring");
+  if ((reinterpret_cast<uintptr_t>(ring) & 7) != 0) return fail(HFX_E_INVALID, "ring must be 8-byte aligned");
+  return HFX_OK;
+}
+
+static hfx_status run_dft(hfx_plan *p, const uint16_t *ring, int64_t ring_frames, int64_t first,
+                          float2 *planes, hipStream_t st) {
+  hfx::DftLaunch L{};
+  L.ring = ring;
+  L.ring_frames = ring_frames;
+  L.first = first;
+  L.T = p->d.window;
+  L.npix = p->npix;
+  L.W = p->W;
+  L.kb = p->d.bin_begin;
+  L.nb = p->nb;
+  L.nbc = p->nbc;
+  L.both = (p->d.flags & HFX_FLAG_BOTH_SIDEBANDS) ? 1 : 0;
+  L.planes = planes;
+  L.lx = p->d.prop == HFX_PROP_FRESNEL ? p->lx : nullptr;
+  L.ly = p->d.prop == HFX_PROP_FRESNEL ? p->ly : nullptr;
+  L.precision = p->d.precision;
+  L.wm_f32 = p->wm32;
+  L.wm_f16hi = reinterpret_cast<const uint16_t *>(p->wm16hi);
+  L.wm_f16lo = reinterpret_cast<const uint16_t *>(p->wm16lo);
+  L.wm_bf16 = reinterpret_cast<const uint16_t *>(p->wmbf);
+  int rc = hfx::launch_time_dft(L, st);
+  if (rc != 0) return fail(HFX_E_HIP, "time DFT launch failed (" + std::to_string(rc) + ")");
+  return HFX_OK;
+}
+
+hfx_status hfx_plan_time_dft(hfx_plan *p, const uint16_t *ring, int64_t ring_frames,
+                             int64_t first_frame, void *planes_out, hfx_stream_t stream) {
+  HFX_TRY(check_ring(p, ring, ring_frames, first_frame));
+  if (!planes_out || (reinterpret_cast<uintptr_t>(planes_out) & 15) != 0)
+    return fail(HFX_E_INVALID, "planes_out must be a 16-byte aligned device buffer");
+  return run_dft(p, ring, ring_frames, first_frame, reinterpret_cast<float2 *>(planes_out),
+                 S(stream));
+}
+
+static hfx::PassArgs base_pass(int H, int W, int planes, int blk_c) {
+  hfx::PassArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.H = H;
+  a.W = W;
+  a.planes = planes;
+  a.blk_c = blk_c;
+  a.in_h = H;
+  a.in_w = W;
+  return a;
+}
+
+static hfx_status window_moments(hfx_plan *p, const uint16_t *ring, int64_t ring_frames,
+                                 int64_t first, float *out, hipStream_t st) {
+  HFX_TRY(run_dft(p, ring, ring_frames, first, p->planes_nat, st));
+  const long long on = p->onpix;
+  hfx::MomentArgs mom{};
+  for (int i = 0; i < p->n_out; ++i) mom.out[i] = out + i * on;
+  mom.f = p->wf;
+  mom.f2 = p->wf2;
+  mom.split = p->d.split_bin > 0 ? p->d.split_bin - p->d.bin_begin : 0;
+  mom.power2 = (p->d.flags & HFX_FLAG_POWER2) ? 1 : 0;
+  mom.shift = (p->d.flags & HFX_FLAG_FFTSHIFT) ? 1 : 0;
+
+  // row pass: natural planes (lens already applied for Fresnel) -> blocked spectrum rows
+  hfx::PassArgs r = base_pass(p->OH, p->OW, p->n_planes, p->blk_c);
+  r.in = p->planes_nat;
+  r.in_mode = hfx::IN_NAT_C;
+  r.in_h = p->H;
+  r.in_w = p->W;
+  r.off_y = (p->OH - p->H) / 2;
+  r.off_x = (p->OW - p->W) / 2;
+  r.out = p->planes_blk;
+  r.out_mode = hfx::OUT_BLK;
+  r.tw = p->tw_w;
+  if (hfx::launch_row_pass(r, st) != 0) return fail(HFX_E_HIP, "row pass launch failed");
+
+  hfx::PassArgs c = base_pass(p->OH, p->OW, p->n_planes, p->blk_c);
+  c.in = p->planes_blk;
+  c.in_mode = hfx::IN_BLK;
+  c.tw = p->tw_h;
+  if (p->d.prop == HFX_PROP_FRESNEL) {
+    c.out_mode = hfx::OUT_MOMENTS;
+    c.mom = mom;
+    if (hfx::launch_col_pass(c, st) != 0) return fail(HFX_E_HIP, "column pass launch failed");
+  } else {
+    c.mid_tf = p->tf;
+    c.out = p->planes_blk;
+    c.out_mode = hfx::OUT_BLK;
+    if (hfx::launch_col_pass(c, st) != 0) return fail(HFX_E_HIP, "column pass launch failed");
+    hfx::PassArgs r2 = base_pass(p->OH, p->OW, p->n_planes, p->blk_c);
+    r2.in = p->planes_blk;
+    r2.in_mode = hfx::IN_BLK;
+    r2.inv = 1;
+    r2.out_mode = hfx::OUT_MOMENTS;
+    r2.mom = mom;
+    r2.tw = p->tw_w;
+    if (hfx::launch_row_pass(r2, st) != 0) return fail(HFX_E_HIP, "inverse row pass launch failed");
+  }
+  return HFX_OK;
+}
+
+hfx_status hfx_moments(hfx_plan *p, const uint16_t *ring, int64_t ring_frames,
+                       int64_t first_frame, float *out, hfx_stream_t stream) {
+  HFX_TRY(check_ring(p, ring, ring_frames, first_frame));
+  if (!out) return fail(HFX_E_INVALID, "null output");
+  return window_moments(p, ring, ring_frames, first_frame, out, S(stream));
+}
+
+hfx_status hfx_moments_batch(hfx_plan *p, const uint16_t *ring, int64_t ring_frames,
+                             int64_t first_frame, int32_t n_windows, int64_t out_stride,
+                             float *out, hfx_stream_t stream) {
+  HFX_TRY(check_ring(p, ring, ring_frames, first_frame));
+  if (!out || n_windows < 0) return fail(HFX_E_INVALID, "invalid output");
+  if (out_stride < (int64_t)p->n_out * p->onpix) return fail(HFX_E_INVALID, "out_stride too small");
+  for (int w = 0; w < n_windows; ++w) {
+    int64_t f = (first_frame + (int64_t)w * p->d.stride) % ring_frames;
+    HFX_TRY(window_moments(p, ring, ring_frames, f, out + w * out_stride, S(stream)));
+  }
+  return HFX_OK;
+}
+
+// ------------------------------------------------------------------------ per-node kinds
+
+hfx_status hfx_dft_axis(const void *in, int32_t in_dtype, int64_t outer, int32_t n, int64_t inner,
+                        int32_t k_begin, int32_t k_end, void *out, hfx_stream_t stream) {
+  if (!in || !out) return fail(HFX_E_INVALID, "null argument");
+  if (in_dtype < 1 || in_dtype > 3) return fail(HFX_E_INVALID, "unsupported input dtype");
+  if (n <= 0 || outer < 0 || inner < 0) return fail(HFX_E_INVALID, "invalid shape");
+  if (k_begin < 0 || k_end < k_begin || k_end > n) return fail(HFX_E_INVALID, "invalid bin range");
+  if (hfx::launch_dft_axis(in, in_dtype, outer, n, inner, k_begin, k_end,
+                           reinterpret_cast<float2 *>(out), S(stream)) != 0)
+    return fail(HFX_E_HIP, "dft launch failed");
+  return HFX_OK;
+}
+
+static hfx_status fresnel_tables(hfx_fresnel *f, hipStream_t st) {
+  hfx::launch_fresnel_chirp(f->lx, f->ly, f->H, f->W, f->lambda, f->z, f->dx, f->dy, st);
+  HFX_TRY_HIP(hipGetLastError());
+  return HFX_OK;
+}
+
+hfx_status hfx_fresnel_create(int64_t batch, int32_t height, int32_t width, double lambda_m,
+                              double dx_m, double dy_m, double z_m, int32_t in_real,
+                              int32_t output_magnitude, int32_t skip_phase_shift,
+                              hfx_stream_t stream, hfx_fresnel **out) {
+  if (!out) return fail(HFX_E_INVALID, "null argument");
+  *out = nullptr;
+  if (batch <= 0 || height <= 0 || width <= 0) return fail(HFX_E_INVALID, "invalid shape");
+  if (!(lambda_m > 0)) return fail(HFX_E_INVALID, "wavelength must be positive");
+  if (!(dx_m > 0) || !(dy_m > 0)) return fail(HFX_E_INVALID, "pixel pitch must be positive");
+  if (dx_m != dy_m) return fail(HFX_E_INVALID, "dx must equal dy");
+  auto *f = new (std::nothrow) hfx_fresnel();
+  if (!f) return fail(HFX_E_NOMEM, "host allocation failed");
+  std::unique_ptr<hfx_fresnel> guard(f);
+  f->batch = batch;
+  f->H = height;
+  f->W = width;
+  f->in_real = in_real;
+  f->output_magnitude = output_magnitude;
+  f->skip_phase_shift = skip_phase_shift;
+  f->lambda = lambda_m;
+  f->dx = dx_m;
+  f->dy = dy_m;
+  f->z = z_m;
+  f->blk_c = hfx::pick_blk_c(width, height);
+  if (!hfx::row_pass_supported(width) || !hfx::col_pass_supported(height, f->blk_c))
+    return fail(HFX_E_UNSUPPORTED, "FFT size must be one of the compiled 2^a*3^b lengths (2..2048)");
+  hipStream_t st = S(stream);
+  HFX_TRY(f->arena.alloc(&f->tw_w, width));
+  HFX_TRY(f->arena.alloc(&f->tw_h, height));
+  HFX_TRY(f->arena.alloc(&f->lx, width));
+  HFX_TRY(f->arena.alloc(&f->ly, height));
+  HFX_TRY(f->arena.alloc(&f->ws, (size_t)batch * height * width));
+  hfx::launch_fft_twiddles(f->tw_w, width, st);
+  hfx::launch_fft_twiddles(f->tw_h, height, st);
+  HFX_TRY(fresnel_tables(f, st));
+  HFX_TRY_HIP(hipStreamSynchronize(st));
+  *out = guard.release();
+  return HFX_OK;
+}
+
+hfx_status hfx_fresnel_update_z(hfx_fresnel *f, double z_m, hfx_stream_t stream) {
+  if (!f) return fail(HFX_E_INVALID, "null argument");
+  f->z = z_m;
+  return fresnel_tables(f, S(stream));
+}
+
+hfx_status hfx_fresnel_execute(hfx_fresnel *f, const void *in, void *out, hfx_stream_t stream) {
+  if (!f || !in || !out) return fail(HFX_E_INVALID, "null argument");
+  hipStream_t st = S(stream);
+  hfx::PassArgs r = base_pass(f->H, f->W, (int)f->batch, f->blk_c);
+  r.in = in;
+  r.in_mode = f->in_real ? hfx::IN_NAT_R : hfx::IN_NAT_C;
+  r.pre_lx = f->lx;
+  r.pre_ly = f->ly;
+  r.out = f->ws;
+  r.out_mode = hfx::OUT_BLK;
+  r.tw = f->tw_w;
+  if (hfx::launch_row_pass(r, st) != 0) return fail(HFX_E_HIP, "row pass launch failed");
+  hfx::PassArgs c = base_pass(f->H, f->W, (int)f->batch, f->blk_c);
+  c.in = f->ws;
+  c.in_mode = hfx::IN_BLK;
+  c.out = out;
+  c.tw = f->tw_h;
+  if (f->output_magnitude) {
+    c.out_mode = hfx::OUT_NAT_MAG;
+  } else {
+    c.out_mode = hfx::OUT_NAT_C;
+    if (!f->skip_phase_shift) {
+      c.post_lx = f->lx;
+      c.post_ly = f->ly;
+    }
+  }
+  if (hfx::launch_col_pass(c, st) != 0) return fail(HFX_E_HIP, "column pass launch failed");
+  return HFX_OK;
+}
+
+hfx_status hfx_fresnel_destroy(hfx_fresnel *f) {
+  delete f;
+  return HFX_OK;
+}
+
+hfx_status hfx_asm_create(const hfx_asm_desc *dp, hfx_stream_t stream, hfx_asm **out) {
+  if (!dp || !out) return fail(HFX_E_INVALID, "null argument");
+  *out = nullptr;
+  const hfx_asm_desc &d = *dp;
+  if (d.batch <= 0 || d.height <= 0 || d.width <= 0) return fail(HFX_E_INVALID, "invalid shape");
+  if (!(d.lambda_m > 0)) return fail(HFX_E_INVALID, "wavelength must be positive");
+  if (!(d.dx_m > 0) || !(d.dy_m > 0)) return fail(HFX_E_INVALID, "pixel pitch must be positive");
+  if (d.dx_m != d.dy_m) return fail(HFX_E_INVALID, "dx must equal dy");
+  auto *a = new (std::nothrow) hfx_asm();
+  if (!a) return fail(HFX_E_NOMEM, "host allocation failed");
+  std::unique_ptr<hfx_asm> guard(a);
+  a->d = d;
+  a->OH = d.pad_height > 0 ? d.pad_height : d.height;
+  a->OW = d.pad_width > 0 ? d.pad_width : d.width;
+  if (a->OH < d.height || a->OW < d.width) return fail(HFX_E_INVALID, "padding smaller than input");
+  if ((a->OH - d.height) % 2 || (a->OW - d.width) % 2)
+    return fail(HFX_E_INVALID, "padding margins must be even");
+  a->blk_c = hfx::pick_blk_c(a->OW, a->OH);
+  if (!hfx::row_pass_supported(a->OW) || !hfx::col_pass_supported(a->OH, a->blk_c))
+    return fail(HFX_E_UNSUPPORTED, "FFT size must be one of the compiled 2^a*3^b lengths (2..2048)");
+  hipStream_t st = S(stream);
+  HFX_TRY(a->arena.alloc(&a->tw_w, a->OW));
+  HFX_TRY(a->arena.alloc(&a->tw_h, a->OH));
+  HFX_TRY(a->arena.alloc(&a->tf, (size_t)a->OH * a->OW));
+  HFX_TRY(a->arena.alloc(&a->ws, (size_t)d.batch * a->OH * a->OW));
+  hfx::launch_fft_twiddles(a->tw_w, a->OW, st);
+  hfx::launch_fft_twiddles(a->tw_h, a->OH, st);
+  hfx::launch_asm_transfer(a->tf, a->OH, a->OW, d.lambda_m, d.z_m, d.dx_m, d.dy_m, d.filter_enabled,
+                           d.r_inner, d.r_outer, d.s_inner, d.s_outer, a->blk_c, st);
+  HFX_TRY_HIP(hipGetLastError());
+  HFX_TRY_HIP(hipStreamSynchronize(st));
+  *out = guard.release();
+  return HFX_OK;
+}
+
+hfx_status hfx_asm_execute(hfx_asm *a, const void *in, void *out, hfx_stream_t stream) {
+  if (!a || !in || !out) return fail(HFX_E_INVALID, "null argument");
+  hipStream_t st = S(stream);
+  const int B = (int)a->d.batch;
+  hfx::PassArgs r = base_pass(a->OH, a->OW, B, a->blk_c);
+  r.in = in;
+  r.in_mode = a->d.in_real ? hfx::IN_NAT_R : hfx::IN_NAT_C;
+  r.in_h = a->d.height;
+  r.in_w = a->d.width;
+  r.off_y = (a->OH - a->d.height) / 2;
+  r.off_x = (a->OW - a->d.width) / 2;
+  r.out = a->ws;
+  r.out_mode = hfx::OUT_BLK;
+  r.tw = a->tw_w;
+  if (hfx::launch_row_pass(r, st) != 0) return fail(HFX_E_HIP, "row pass launch failed");
+  hfx::PassArgs c = base_pass(a->OH, a->OW, B, a->blk_c);
+  c.in = a->ws;
+  c.in_mode = hfx::IN_BLK;
+  c.mid_tf = a->tf;
+  c.out = a->ws;
+  c.out_mode = hfx::OUT_BLK;
+  c.tw = a->tw_h;
+  if (hfx::launch_col_pass(c, st) != 0) return fail(HFX_E_HIP, "column pass launch failed");
+  hfx::PassArgs r2 = base_pass(a->OH, a->OW, B, a->blk_c);
+  r2.in = a->ws;
+  r2.in_mode = hfx::IN_BLK;
+  r2.inv = 1;
+  r2.out = out;
+  r2.out_mode = hfx::OUT_NAT_C;
+  r2.tw = a->tw_w;
+  if (hfx::launch_row_pass(r2, st) != 0) return fail(HFX_E_HIP, "inverse row pass launch failed");
+  return HFX_OK;
+}
+
+hfx_status hfx_asm_destroy(hfx_asm *a) {
+  delete a;
+  return HFX_OK;
+}
+
+hfx_status hfx_weighted_mean(const void *in, int32_t in_complex, int64_t planes, int64_t npix,
+                             const float *weights, int32_t power2, float *out,
+                             hfx_stream_t stream) {
+  if (!in || !out) return fail(HFX_E_INVALID, "null argument");
+  if (planes <= 0 || npix <= 0) return fail(HFX_E_INVALID, "reduction has zero elements");
+  float *outs[3] = {out, nullptr, nullptr};
+  int rc;
+  if (weights) {
+    // out = mean_k(w_k * m_k): route the weights through the M1 slot of the reducer and
+    // discard M0 into the same buffer first.
+    outs[0] = out;
+    outs[1] = out;
+    rc = hfx::launch_moments_reduce(in, in_complex, planes, npix, weights, weights, outs, 2, power2,
+                                    S(stream));
+  } else {
+    rc = hfx::launch_moments_reduce(in, in_complex, planes, npix, nullptr, nullptr, outs, 1, power2,
+                                    S(stream));
+  }
+  if (rc != 0) return fail(HFX_E_HIP, "reduce launch failed");
+  return HFX_OK;
+}
+
+hfx_status hfx_fftshift(const void *in, void *out, int32_t elem_size, int32_t ndim,
+                        const int64_t *shape, const int32_t *axes_mask, hfx_stream_t stream) {
+  if (!in || !out || !shape || !axes_mask) return fail(HFX_E_INVALID, "null argument");
+  if (in == out) return fail(HFX_E_INVALID, "fftshift is out of place");
+  if (ndim < 1 || ndim > 8) return fail(HFX_E_INVALID, "rank must be in [1, 8]");
+  if (elem_size <= 0) return fail(HFX_E_INVALID, "element size must be positive");
+  long long shp[8];
+  int msk[8];
+  for (int i = 0; i < ndim; ++i) {
+    if (shape[i] < 0) return fail(HFX_E_INVALID, "negative dimension");
+    shp[i] = shape[i];
+    msk[i] = axes_mask[i];
+  }
+  if (hfx::launch_fftshift(in, out, elem_size, ndim, shp, msk, S(stream)) != 0)
+    return fail(HFX_E_HIP, "fftshift launch failed");
+  return HFX_OK;
+}
+
+hfx_status hfx_convert_u16(const uint16_t *in, void *out, int64_t n, int32_t to_complex,
+                           hfx_stream_t stream) {
+  if ((!in || !out) && n > 0) return fail(HFX_E_INVALID, "null argument");
+  if (hfx::launch_convert_u16(in, out, n, to_complex, S(stream)) != 0)
+    return fail(HFX_E_HIP, "conversion launch failed");
+  return HFX_OK;
+}
+
+void hfx_synth_desc_init(hfx_synth_desc *d, int32_t height, int32_t width) {
+  std::memset(d, 0, sizeof(*d));
+  d->height = height;
+  d->width = width;
+  d->disk_radius_px = 0.39 * (height < width ? height : width);
+  d->fs_hz = 37e3;
+  d->ref_amp = 1.0;
+  d->obj_sigma = 0.5;
+  d->gain = 1000.0;
+  d->carrier_cyc_per_px = 0.25;
+  d->phase_noise_rad = 0.3;
+  d->fd_max_hz = 15e3;
+  d->seed = 1234u;
+  d->stream_id = 0u;
+}
+
+hfx_status hfx_synth_frames(const hfx_synth_desc *d, uint16_t *frames, int64_t first_frame,
+                            int64_t n_frames, double *phase_state, hfx_stream_t stream) {
+  if (!d || !frames || !phase_state) return fail(HFX_E_INVALID, "null argument");
+  if (d->height <= 0 || d->width <= 0 || n_frames < 0) return fail(HFX_E_INVALID, "invalid shape");
+  if (!(d->fs_hz > 0)) return fail(HFX_E_INVALID, "fs must be positive");
+  hfx::SynthArgs a;
+  a.H = d->height;
+  a.W = d->width;
+  a.disk_radius = d->disk_radius_px;
+  a.fs = d->fs_hz;
+  a.ref_amp = d->ref_amp;
+  a.obj_sigma = d->obj_sigma;
+  a.gain = d->gain;
+  a.carrier = d->carrier_cyc_per_px;
+  a.phase_noise = d->phase_noise_rad;
+  a.fd_max = d->fd_max_hz;
+  a.seed = d->seed;
+  a.stream_id = d->stream_id;
+  if (hfx::launch_synth(a, frames, first_frame, n_frames, phase_state, S(stream)) != 0)
+    return fail(HFX_E_HIP, "synth launch failed");
+  return HFX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,106 @@
+// Host-side launch interface of the gfx950 kernels.  Everything here is internal to
+// libhfx; the public surface is the C ABI in include/hfx.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hfx_device.hh"
+
+struct __half;
+struct __hip_bfloat16;
+
+namespace hfx {
+
+// ---- FP64 table generation (tables.hip)
+void launch_fft_twiddles(float2 *tw, int n, hipStream_t st);
+void launch_dft_matrix(float *f32, __half *f16hi, __half *f16lo, __hip_bfloat16 *bf16, int rows,
+                       int T, int kb, int nbc, hipStream_t st);
+void launch_fresnel_chirp(float2 *lx, float2 *ly, int H, int W, double lambda, double z, double dx,
+                          double dy, hipStream_t st);
+void launch_asm_transfer(float2 *out, int H, int W, double lambda, double z, double dx, double dy,
+                         int filt, double r_in, double r_out, double s_in, double s_out, int blk_c,
+                         hipStream_t st);
+
+// ---- time DFT (dft.hip)
+struct DftLaunch {
+  const uint16_t *ring;
+  long long ring_frames, first;
+  int T;
+  long long npix;
+  int W;
+  int kb, nb, nbc, both;
+  float2 *planes;
+  const float2 *lx, *ly;
+  int precision;  // 0 FP32, 1 FP16, 2 BF16, 3 FP16 split
+  const float *wm_f32;
+  const uint16_t *wm_f16hi, *wm_f16lo, *wm_bf16;
+};
+int launch_time_dft(const DftLaunch &L, hipStream_t st);
+
+// ---- 2-D FFT passes (fft2d.hip)
+enum InMode { IN_NAT_C = 0, IN_NAT_R = 1, IN_BLK = 2 };
+enum OutMode { OUT_BLK = 0, OUT_NAT_C = 1, OUT_NAT_MAG = 2, OUT_MOMENTS = 3 };
+
+struct MomentArgs {
+  float *out[5];       // M0, M1, M2, M0 low band, M0 high band (out[3..4] may be null)
+  const float *f;      // [planes] weights f_k (FP32, arange semantics)
+  const float *f2;     // [planes] f_k * f_k (FP32)
+  int split;           // band split plane index (0 = none)
+  int power2;          // accumulate |X|^2 instead of |X|
+  int shift;           // fftshift the stored images on both axes
+};
+
+struct PassArgs {
+  int H, W;                 // transform grid (output plane size)
+  int planes;               // number of planes
+  int inv;                  // inverse transform (exp(+i...)), unnormalised
+  int blk_c;                // column-block width of the blocked layout
+  // input
+  const void *in;
+  int in_mode;
+  int in_h, in_w;           // input plane dims (== H, W unless zero-padded)
+  int off_y, off_x;         // centred padding offsets
+  const float2 *pre_lx, *pre_ly;  // lens applied to the input (row pass) or null
+  // column pass: multiply by a blocked table between forward and inverse column FFTs
+  const float2 *mid_tf;
+  // output
+  void *out;
+  int out_mode;
+  const float2 *post_lx, *post_ly;  // lens applied to the output or null
+  MomentArgs mom;
+  const float2 *tw;         // FFT twiddles for the transform length
+  const float2 *tw2;        // second twiddle table unused by single-axis passes
+};
+
+int launch_row_pass(const PassArgs &a, hipStream_t st);
+int launch_col_pass(const PassArgs &a, hipStream_t st);
+bool row_pass_supported(int W);
+bool col_pass_supported(int H, int blk_c);
+int pick_blk_c(int W, int H);
+
+// ---- small ops (ops.hip)
+int launch_fftshift(const void *in, void *out, int elem_size, int ndim, const long long *shape,
+                    const int *shift_axes_mask, hipStream_t st);
+int launch_moments_reduce(const void *in, int in_complex, long long planes, long long npix,
+                          const float *f, const float *f2, float **out, int n_out, int power2,
+                          hipStream_t st);
+int launch_convert_u16(const uint16_t *in, void *out, long long n, int to_complex, hipStream_t st);
+int launch_dft_axis(const void *in, int dtype, long long outer, int n, long long inner, int kb,
+                    int ke, float2 *out, hipStream_t st);
+
+// ---- synthetic interferograms (synth.hip)
+struct SynthArgs {
+  int H, W;
+  double disk_radius;
+  double fs;
+  double ref_amp, obj_sigma, gain;
+  double carrier;
+  double phase_noise;
+  double fd_max;
+  uint32_t seed, stream_id;
+};
+int launch_synth(const SynthArgs &a, uint16_t *frames, long long first_frame, long long n_frames,
+                 double *phase_state, hipStream_t st);
+
+}  // namespace hfx

@@ -1,0 +1,260 @@
+"""Python host API over libhfx's C ABI (device memory and streams come from torch).
+
+Every call goes through the HIP library; nothing here computes on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import torch
+
+from . import _lib
+from ._lib import check, lib
+
+
+def _stream(stream) -> C.c_void_p:
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    if isinstance(stream, torch.cuda.Stream):
+        return C.c_void_p(stream.cuda_stream)
+    return C.c_void_p(int(stream))
+
+
+def _ptr(t: torch.Tensor) -> C.c_void_p:
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+@dataclass
+class MomentConfig:
+    """Settings of the fused Doppler-moment plan (mirror of hfx_plan_desc)."""
+
+    height: int
+    width: int
+    window: int
+    stride: int
+    bin_begin: int
+    bin_end: int
+    prop: str = "fresnel"          # "fresnel" | "asm"
+    lambda_m: float = 852e-9
+    dx_m: float = 12e-6
+    dy_m: float = 12e-6
+    z_m: float = 0.3
+    fs_hz: float = 37e3
+    weight_fs_hz: float = 37e3
+    precision: str = "fp32"        # fp32 | fp16 | bf16 | fp16_split
+    fftshift: bool = False
+    power2: bool = False
+    both_sidebands: bool = False
+    pad: tuple = (0, 0)
+    filt: Optional[tuple] = None
+    split_bin: int = 0
+
+    def to_desc(self) -> _lib.PlanDesc:
+        d = _lib.PlanDesc()
+        lib().hfx_plan_desc_init(C.byref(d))
+        d.height, d.width = self.height, self.width
+        d.window, d.stride = self.window, self.stride
+        d.fs_hz = self.fs_hz
+        d.lambda_m, d.dx_m, d.dy_m, d.z_m = self.lambda_m, self.dx_m, self.dy_m, self.z_m
+        d.bin_begin, d.bin_end = self.bin_begin, self.bin_end
+        d.prop = {"fresnel": _lib.PROP_FRESNEL, "asm": _lib.PROP_ANGULAR_SPECTRUM}[self.prop]
+        d.precision = _lib.PRECISIONS[self.precision]
+        d.flags = ((_lib.FLAG_FFTSHIFT if self.fftshift else 0)
+                   | (_lib.FLAG_POWER2 if self.power2 else 0)
+                   | (_lib.FLAG_BOTH_SIDEBANDS if self.both_sidebands else 0))
+        d.pad_height, d.pad_width = int(self.pad[0]), int(self.pad[1])
+        if self.filt is not None:
+            d.filter_enabled = 1
+            d.r_inner, d.r_outer, d.s_inner, d.s_outer = (float(v) for v in self.filt)
+        d.split_bin = self.split_bin
+        d.weight_fs_hz = self.weight_fs_hz
+        return d
+
+
+class MomentPlan:
+    """Resident plan of the fused path: tables and workspaces live on the device."""
+
+    def __init__(self, cfg: MomentConfig, stream=None):
+        self.cfg = cfg
+        self._h = C.c_void_p()
+        d = cfg.to_desc()
+        check(lib().hfx_plan_create(C.byref(d), _stream(stream), C.byref(self._h)))
+        info = _lib.PlanInfo()
+        check(lib().hfx_plan_get_info(self._h, C.byref(info)))
+        self.out_shape = (info.n_outputs, info.out_height, info.out_width)
+        self.n_planes = info.n_planes
+        self.workspace_bytes = info.workspace_bytes
+
+    def close(self):
+        if self._h:
+            lib().hfx_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    __del__ = close
+
+    def update_z(self, z_m: float, stream=None):
+        check(lib().hfx_plan_update_z(self._h, z_m, _stream(stream)))
+        self.cfg.z_m = z_m
+
+    def new_output(self, n_windows: int = 0) -> torch.Tensor:
+        shape = self.out_shape if n_windows == 0 else (n_windows,) + self.out_shape
+        return torch.empty(shape, dtype=torch.float32, device="cuda")
+
+    def moments(self, ring: torch.Tensor, first_frame: int = 0, out: Optional[torch.Tensor] = None,
+                stream=None) -> torch.Tensor:
+        """ring: device uint16 [F][H][W] (int16 storage is accepted bit-for-bit)."""
+        out = self.new_output() if out is None else out
+        check(lib().hfx_moments(self._h, _ptr(ring), ring.shape[0], first_frame, _ptr(out),
+                                _stream(stream)))
+        return out
+
+    def moments_batch(self, ring: torch.Tensor, first_frame: int, n_windows: int,
+                      out: Optional[torch.Tensor] = None, stream=None) -> torch.Tensor:
+        out = self.new_output(n_windows) if out is None else out
+        stride = out[0].numel() if n_windows else 0
+        check(lib().hfx_moments_batch(self._h, _ptr(ring), ring.shape[0], first_frame, n_windows,
+                                      stride, _ptr(out), _stream(stream)))
+        return out
+
+    def time_dft(self, ring: torch.Tensor, first_frame: int = 0, stream=None) -> torch.Tensor:
+        """Runs the time-DFT stage only: [n_planes][H][W] complex64 (lens applied for Fresnel)."""
+        out = torch.empty((self.n_planes, self.cfg.height, self.cfg.width), dtype=torch.complex64,
+                          device="cuda")
+        check(lib().hfx_plan_time_dft(self._h, _ptr(ring), ring.shape[0], first_frame, _ptr(out),
+                                      _stream(stream)))
+        return out
+
+
+# ----------------------------------------------------------------------- per-node kinds
+def dft_axis(x: torch.Tensor, axis: int, k_begin: int = 0, k_end: Optional[int] = None,
+             stream=None) -> torch.Tensor:
+    """RFFT/FFT along `axis` (unscaled), bins [k_begin, k_end) -> complex64."""
+    axis = axis % x.dim()
+    n = x.shape[axis]
+    k_end = n if k_end is None else k_end
+    outer = 1
+    for s in x.shape[:axis]:
+        outer *= s
+    inner = 1
+    for s in x.shape[axis + 1:]:
+        inner *= s
+    dt = {torch.int16: _lib.DTYPE_U16, torch.uint16: _lib.DTYPE_U16, torch.float32: _lib.DTYPE_F32,
+          torch.complex64: _lib.DTYPE_CF32}[x.dtype]
+    x = x.contiguous()
+    shape = list(x.shape)
+    shape[axis] = k_end - k_begin
+    out = torch.empty(shape, dtype=torch.complex64, device="cuda")
+    check(lib().hfx_dft_axis(_ptr(x), dt, outer, n, inner, k_begin, k_end, _ptr(out),
+                             _stream(stream)))
+    return out
+
+
+class Fresnel:
+    def __init__(self, batch, height, width, lambda_m, dx_m, dy_m, z_m, in_real=False,
+                 output_magnitude=False, skip_phase_shift=False, stream=None):
+        self._h = C.c_void_p()
+        self.shape = (batch, height, width)
+        self.output_magnitude = output_magnitude
+        check(lib().hfx_fresnel_create(batch, height, width, lambda_m, dx_m, dy_m, z_m, int(in_real),
+                                       int(output_magnitude), int(skip_phase_shift), _stream(stream),
+                                       C.byref(self._h)))
+
+    def update_z(self, z_m, stream=None):
+        check(lib().hfx_fresnel_update_z(self._h, z_m, _stream(stream)))
+
+    def __call__(self, x: torch.Tensor, out: Optional[torch.Tensor] = None, stream=None):
+        if out is None:
+            dt = torch.float32 if self.output_magnitude else torch.complex64
+            out = torch.empty(x.shape, dtype=dt, device="cuda")
+        check(lib().hfx_fresnel_execute(self._h, _ptr(x), _ptr(out), _stream(stream)))
+        return out
+
+    def close(self):
+        if self._h:
+            lib().hfx_fresnel_destroy(self._h)
+            self._h = C.c_void_p()
+
+    __del__ = close
+
+
+class AngularSpectrum:
+    def __init__(self, batch, height, width, lambda_m, dx_m, dy_m, z_m, in_real=False, pad=(0, 0),
+                 filt=None, stream=None):
+        d = _lib.AsmDesc()
+        d.batch, d.height, d.width = batch, height, width
+        d.lambda_m, d.dx_m, d.dy_m, d.z_m = lambda_m, dx_m, dy_m, z_m
+        d.in_real = int(in_real)
+        d.pad_height, d.pad_width = int(pad[0]), int(pad[1])
+        if filt is not None:
+            d.filter_enabled = 1
+            d.r_inner, d.r_outer, d.s_inner, d.s_outer = (float(v) for v in filt)
+        self.out_shape = (batch, pad[0] or height, pad[1] or width)
+        self._h = C.c_void_p()
+        check(lib().hfx_asm_create(C.byref(d), _stream(stream), C.byref(self._h)))
+
+    def __call__(self, x: torch.Tensor, out: Optional[torch.Tensor] = None, stream=None):
+        if out is None:
+            out = torch.empty(self.out_shape, dtype=torch.complex64, device="cuda")
+        check(lib().hfx_asm_execute(self._h, _ptr(x), _ptr(out), _stream(stream)))
+        return out
+
+    def close(self):
+        if self._h:
+            lib().hfx_asm_destroy(self._h)
+            self._h = C.c_void_p()
+
+    __del__ = close
+
+
+def weighted_mean(x: torch.Tensor, weights: Optional[torch.Tensor] = None, power2=False,
+                  stream=None) -> torch.Tensor:
+    """Mean over axis 0 of |x| (complex) or x (real), optionally weighted per plane."""
+    planes = x.shape[0]
+    npix = x[0].numel()
+    out = torch.empty(x.shape[1:], dtype=torch.float32, device="cuda")
+    w = C.c_void_p(weights.data_ptr()) if weights is not None else None
+    check(lib().hfx_weighted_mean(_ptr(x.contiguous()), int(x.is_complex()), planes, npix, w,
+                                  int(power2), _ptr(out), _stream(stream)))
+    return out
+
+
+def fftshift(x: torch.Tensor, axes: Sequence[int], stream=None) -> torch.Tensor:
+    x = x.contiguous()
+    nd = x.dim()
+    shape = (C.c_int64 * nd)(*x.shape)
+    mask = (C.c_int32 * nd)(*[1 if (i in [a % nd for a in axes]) else 0 for i in range(nd)])
+    out = torch.empty_like(x)
+    check(lib().hfx_fftshift(_ptr(x), _ptr(out), x.element_size(), nd, shape, mask, _stream(stream)))
+    return out
+
+
+def convert_u16(x: torch.Tensor, to_complex=False, stream=None) -> torch.Tensor:
+    out = torch.empty(x.shape, dtype=torch.complex64 if to_complex else torch.float32, device="cuda")
+    check(lib().hfx_convert_u16(_ptr(x.contiguous()), _ptr(out), x.numel(), int(to_complex),
+                                _stream(stream)))
+    return out
+
+
+def synth_desc(height, width, **kw) -> _lib.SynthDesc:
+    d = _lib.SynthDesc()
+    lib().hfx_synth_desc_init(C.byref(d), height, width)
+    for k, v in kw.items():
+        setattr(d, k, v)
+    return d
+
+
+def synth_frames(desc: _lib.SynthDesc, first: int, n: int, out: Optional[torch.Tensor] = None,
+                 phase_state: Optional[torch.Tensor] = None, stream=None):
+    """Generates frames [first, first+n) into out ([n][H][W] int16 storage of uint16)."""
+    H, W = desc.height, desc.width
+    if out is None:
+        out = torch.empty((n, H, W), dtype=torch.int16, device="cuda")
+    if phase_state is None:
+        phase_state = torch.zeros(H * W, dtype=torch.float64, device="cuda")
+    check(lib().hfx_synth_frames(C.byref(desc), _ptr(out), first, n, _ptr(phase_state),
+                                 _stream(stream)))
+    return out, phase_state

@@ -1,0 +1,191 @@
+/*
+ * hfx.h - C ABI of the MI355X (gfx950) Doppler-moment reconstruction path.
+ *
+ * Everything is plain C: device pointers, sizes and a stream handle (a hipStream_t passed
+ * as void*).  No exception crosses this boundary: every entry point returns an hfx_status
+ * and hfx_last_error() returns the thread-local message of the last failure.  The C++
+ * holoflow node layer (include/holoflow/...) maps HFX_E_INVALID to std::invalid_argument
+ * and HFX_E_HIP to curaii::HipError, as the reference's factories throw
+ * std::invalid_argument from infer (fresnel_diffraction.cu:94-99) and curaii::CudaError
+ * from the runtime (curaii/cuda.hh:29-36).
+ *
+ * Plans own their device tables and workspaces; they are allocated in *_create and freed
+ * in *_destroy.  *_execute / hfx_moments* never allocate, never synchronise and only
+ * enqueue work on the given stream, so they can be captured into a HIP graph.  A plan is
+ * used by one thread / stream at a time; distinct plans may run concurrently.
+ */
+#ifndef HFX_H_
+#define HFX_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int hfx_status;
+#define HFX_OK 0
+#define HFX_E_INVALID 1     /* invalid argument (reference: std::invalid_argument) */
+#define HFX_E_UNSUPPORTED 2 /* valid but outside what the kernels implement */
+#define HFX_E_HIP 3         /* HIP runtime failure (reference: curaii::CudaError) */
+#define HFX_E_NOMEM 4       /* device allocation failed */
+
+typedef void *hfx_stream_t; /* a hipStream_t; NULL is the default stream */
+
+/* Thread-local text of the last failure on this thread ("" if none). */
+const char *hfx_last_error(void);
+/* ABI version, bumped on any incompatible change. */
+int32_t hfx_abi_version(void);
+
+/* ------------------------------------------------------------------------------------
+ * Fused Doppler-moment plan: uint16 window -> time DFT (band) -> Fresnel or angular-
+ * spectrum propagation -> |X| -> M0/M1/M2 (+ band-split M0) -> optional fftshift.
+ * Replaces the node chain GraphBuilder wires in build_time_frequency_analysis
+ * (graph_builder.cc:245-305), build_spatial_propagation (:522-566) and build_xy_view
+ * (:604-647): Conversion -> RFFT/FFT -> Slice/Copy/Concatenate -> FresnelDiffraction |
+ * AngularSpectrum -> MeanAbs / Abs,Arange,Multiply,Mean -> FFTShift -> Mean(0).
+ * ---------------------------------------------------------------------------------- */
+
+enum hfx_prop { HFX_PROP_FRESNEL = 0, HFX_PROP_ANGULAR_SPECTRUM = 1 };
+enum hfx_precision {
+  HFX_PREC_FP32 = 0,      /* FP32 MFMA time DFT */
+  HFX_PREC_FP16 = 1,      /* FP16 MFMA, FP32 accumulate */
+  HFX_PREC_BF16 = 2,      /* BF16 MFMA, FP32 accumulate */
+  HFX_PREC_FP16_SPLIT = 3 /* FP16 MFMA with hi/lo split twiddles, FP32 accumulate */
+};
+enum hfx_flags {
+  HFX_FLAG_FFTSHIFT = 1u << 0,       /* fftshift the moment images (pp_fft_shift) */
+  HFX_FLAG_POWER2 = 1u << 1,         /* integrate |X|^2 instead of the reference's |X| */
+  HFX_FLAG_BOTH_SIDEBANDS = 1u << 2, /* TimeMethod::FFT: positive and negative band planes */
+};
+
+typedef struct hfx_plan_desc {
+  int32_t height, width;    /* frame size H x W */
+  int32_t window, stride;   /* STFT window T and stride S (frames) */
+  double fs_hz;             /* camera frame rate (informational unless weight_fs_hz <= 0) */
+  double lambda_m, dx_m, dy_m, z_m;
+  int32_t bin_begin, bin_end; /* band bins [begin, end) of the T-point DFT, end <= T/2+1 */
+  int32_t prop;             /* enum hfx_prop */
+  int32_t precision;        /* enum hfx_precision */
+  uint32_t flags;           /* enum hfx_flags */
+  int32_t pad_height, pad_width; /* angular spectrum centred zero padding, 0 = none */
+  int32_t filter_enabled;   /* angular spectrum annular filter */
+  double r_inner, r_outer, s_inner, s_outer;
+  int32_t split_bin;        /* >0: also M0 over [begin, split) and [split, end) */
+  double weight_fs_hz;      /* sampling rate of the M1/M2 weights f_k = k*fs/T; the
+                               reference hard-codes 37000 (graph_builder.cc:580) */
+} hfx_plan_desc;
+
+typedef struct hfx_plan_info {
+  int32_t out_height, out_width; /* moment image size (padded size for ASM padding) */
+  int32_t n_outputs;             /* 3, or 5 with a band split */
+  int32_t n_planes;              /* propagated planes (nb, or 2*nb for both sidebands) */
+  int64_t workspace_bytes;       /* device bytes owned by the plan */
+} hfx_plan_info;
+
+typedef struct hfx_plan hfx_plan;
+
+/* Fills a descriptor with the reference defaults (fs weights 37 kHz, FP32, Fresnel). */
+void hfx_plan_desc_init(hfx_plan_desc *d);
+/* Validates, allocates tables/workspaces and generates the FP64 phase tables on `stream`
+ * (synchronised before return).  Replaces FresnelDiffractionFactory::create
+ * (fresnel_diffraction.cu:510-632) / AngularSpectrumFactory::create
+ * (angular_spectrum.cu:496-580) plus RFFTFactory::create (rfft.cu:264-297) for the path. */
+hfx_status hfx_plan_create(const hfx_plan_desc *d, hfx_stream_t stream, hfx_plan **out);
+hfx_status hfx_plan_destroy(hfx_plan *p);
+hfx_status hfx_plan_get_info(const hfx_plan *p, hfx_plan_info *info);
+/* Rewrites the propagation tables for a new z only (the reference's z-only reuse,
+ * FresnelDiffractionFactory::update, fresnel_diffraction.cu:634-668). */
+hfx_status hfx_plan_update_z(hfx_plan *p, double z_m, hfx_stream_t stream);
+/* One window: frames first_frame .. first_frame+T-1 (mod ring_frames) of a device ring of
+ * ring_frames frames of H*W uint16 each.  `out` is a device buffer of n_outputs*OH*OW
+ * floats: M0, M1, M2 [, M0 low band, M0 high band]. */
+hfx_status hfx_moments(hfx_plan *p, const uint16_t *ring, int64_t ring_frames,
+                       int64_t first_frame, float *out, hfx_stream_t stream);
+/* n_windows consecutive windows starting at first_frame, advancing by the plan stride;
+ * window w writes out + w * out_stride floats. */
+hfx_status hfx_moments_batch(hfx_plan *p, const uint16_t *ring, int64_t ring_frames,
+                             int64_t first_frame, int32_t n_windows, int64_t out_stride,
+                             float *out, hfx_stream_t stream);
+/* Time-DFT stage of the plan alone (RFFT -> band slice, with the Fresnel lens applied
+ * when the plan propagates with Fresnel): writes n_planes complex64 planes [n_planes][H][W]
+ * to planes_out (device). */
+hfx_status hfx_plan_time_dft(hfx_plan *p, const uint16_t *ring, int64_t ring_frames,
+                             int64_t first_frame, void *planes_out, hfx_stream_t stream);
+
+/* ------------------------------------------------------------------------------------
+ * Per-node kinds (drop-in replacements of individual reference tasks).
+ * ---------------------------------------------------------------------------------- */
+
+/* holonp::RFFT / holonp::FFT along one axis (rfft.cu:148-297, fft.cu:234-386), norm
+ * "backward" (unscaled), for a contiguous tensor viewed as [outer][n][inner] with the
+ * transform over n.  in_dtype: 1 = U16, 2 = F32, 3 = CF32 (FFT only).  Outputs CF32
+ * [outer][n_out][inner] holding bins [k_begin, k_end) (RFFT: k_end <= n/2+1). */
+hfx_status hfx_dft_axis(const void *in, int32_t in_dtype, int64_t outer, int32_t n, int64_t inner,
+                        int32_t k_begin, int32_t k_end, void *out, hfx_stream_t stream);
+
+/* holotask::syncs::FresnelDiffraction (fresnel_diffraction.cu:386-668) over a contiguous
+ * [batch][H][W] CF32 (in_real = 0) or F32 (in_real = 1) input, axes (-2, -1). */
+typedef struct hfx_fresnel hfx_fresnel;
+hfx_status hfx_fresnel_create(int64_t batch, int32_t height, int32_t width, double lambda_m,
+                              double dx_m, double dy_m, double z_m, int32_t in_real,
+                              int32_t output_magnitude, int32_t skip_phase_shift,
+                              hfx_stream_t stream, hfx_fresnel **out);
+hfx_status hfx_fresnel_update_z(hfx_fresnel *f, double z_m, hfx_stream_t stream);
+hfx_status hfx_fresnel_execute(hfx_fresnel *f, const void *in, void *out, hfx_stream_t stream);
+hfx_status hfx_fresnel_destroy(hfx_fresnel *f);
+
+/* holotask::syncs::AngularSpectrum (angular_spectrum.cu:393-580): U = IFFT2_unnorm(
+ * FFT2(pad(u)) * ifftshift(H)) over [batch][H][W] -> CF32 [batch][OH][OW]. */
+typedef struct hfx_asm_desc {
+  int64_t batch;
+  int32_t height, width;
+  double lambda_m, dx_m, dy_m, z_m;
+  int32_t in_real;
+  int32_t pad_height, pad_width; /* 0 = no padding */
+  int32_t filter_enabled;
+  double r_inner, r_outer, s_inner, s_outer;
+} hfx_asm_desc;
+typedef struct hfx_asm hfx_asm;
+hfx_status hfx_asm_create(const hfx_asm_desc *d, hfx_stream_t stream, hfx_asm **out);
+hfx_status hfx_asm_execute(hfx_asm *a, const void *in, void *out, hfx_stream_t stream);
+hfx_status hfx_asm_destroy(hfx_asm *a);
+
+/* holotask::syncs::MeanAbs (mean_abs.cu:231-276) / holonp::Mean (mean.cu:254-306) /
+ * Multiply+Mean (multiply.cu:69-92) over the plane axis of [planes][npix]:
+ *   out[p] = (1/planes) * sum_k w_k * m_k(p),  m = |x| (complex in) or x (real in),
+ * with w_k = weights[k] (device array) or 1 when weights is NULL.  power2: m = |x|^2. */
+hfx_status hfx_weighted_mean(const void *in, int32_t in_complex, int64_t planes, int64_t npix,
+                             const float *weights, int32_t power2, float *out,
+                             hfx_stream_t stream);
+
+/* holonp::FFTShift (fftshift.cu:110-145,254-260): out[i] = in[(i + n/2) mod n] on every
+ * axis whose axes_mask entry is non-zero; elements of elem_size bytes. */
+hfx_status hfx_fftshift(const void *in, void *out, int32_t elem_size, int32_t ndim,
+                        const int64_t *shape, const int32_t *axes_mask, hfx_stream_t stream);
+
+/* holotask::syncs::Conversion U16 -> F32 (real) or CF32 (x + 0i) (conversion.cu:156-202). */
+hfx_status hfx_convert_u16(const uint16_t *in, void *out, int64_t n, int32_t to_complex,
+                           hfx_stream_t stream);
+
+/* ------------------------------------------------------------------------------------
+ * Seeded synthetic interferograms (replaces the camera / holofile sources for tests and
+ * benchmarks; no reference counterpart on the path).  phase_state: device array of
+ * H*W doubles, zero before frame 0, carried between calls that continue a stream.
+ * ---------------------------------------------------------------------------------- */
+typedef struct hfx_synth_desc {
+  int32_t height, width;
+  double disk_radius_px;
+  double fs_hz;
+  double ref_amp, obj_sigma, gain, carrier_cyc_per_px, phase_noise_rad, fd_max_hz;
+  uint32_t seed, stream_id;
+} hfx_synth_desc;
+void hfx_synth_desc_init(hfx_synth_desc *d, int32_t height, int32_t width);
+hfx_status hfx_synth_frames(const hfx_synth_desc *d, uint16_t *frames, int64_t first_frame,
+                            int64_t n_frames, double *phase_state, hfx_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HFX_H_ */

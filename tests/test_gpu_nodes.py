@@ -1,0 +1,147 @@
+"""GPU parity of the per-node kinds against the oracle and the reference's own tests.
+
+Every call goes through libhfx's C ABI (holoflow_amd.hfx wraps it with torch memory)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from oracle.synth import SynthParams, synth_frames
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a, dtype=None):
+    a = np.ascontiguousarray(a)
+    if a.dtype == np.uint16:
+        a = a.view(np.int16)
+    t = torch.from_numpy(a)
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.cuda()
+
+
+def test_synth_gpu_matches_cpu_twin(gpu):
+    from holoflow_amd import hfx
+
+    sp = SynthParams(48, 64, 18.0, seed=99, stream_id=3)
+    d = hfx.synth_desc(48, 64, disk_radius_px=18.0, seed=99, stream_id=3)
+    a, st = hfx.synth_frames(d, 0, 6)
+    b, _ = hfx.synth_frames(d, 6, 4, phase_state=st)
+    g = torch.cat([a, b]).cpu().numpy().view(np.uint16).astype(np.int64)
+    c, _ = synth_frames(sp, 0, 10)
+    diff = np.abs(g - c.astype(np.int64))
+    assert diff.max() <= 1
+    assert (diff != 0).mean() < 1e-3
+
+
+@pytest.mark.parametrize("T,kb,ke,both", [(32, 6, 13, False), (64, 11, 26, False), (64, 0, 5, False),
+                                          (128, 13, 65, False), (64, 11, 26, True), (16, 1, 9, False)])
+def test_time_dft_stage_vs_fp64(gpu, T, kb, ke, both):
+    from holoflow_amd import hfx
+
+    H, W = 24, 48
+    rng = np.random.default_rng(T + kb)
+    ring = rng.integers(0, 4096, size=(T + 7, H, W)).astype(np.uint16)
+    first = 5  # the window wraps around the ring end
+    win = np.concatenate([ring[first:], ring[:first]])[:T]
+    cfg = hfx.MomentConfig(H, W, T, T, kb, ke, "asm", both_sidebands=both)
+    plan = hfx.MomentPlan(cfg)
+    S = plan.time_dft(_dev(ring), first).cpu().numpy()
+    p = O.PathParams(H, W, T, T, kb, ke, both_sidebands=both)
+    ref = O.time_dft(win, p, "f64").numpy()
+    assert rel_l2(S, ref) < 1e-6
+
+
+def test_fresnel_node_vs_oracle_and_reference_magnitude_test(gpu):
+    from holoflow_amd import hfx
+
+    # fresnel_diffraction_test.cu:146-161 shape and input pattern
+    B, H, W = 6, 32, 48
+    i = np.arange(B * H * W)
+    x = ((i % 7) - 3.0 + 1j * ((i % 5) - 2.0)).astype(np.complex64).reshape(B, H, W)
+    lam, dx, z = 852e-9, 20e-6, 0.01
+    fc = hfx.Fresnel(B, H, W, lam, dx, dx, z, output_magnitude=False)
+    fm = hfx.Fresnel(B, H, W, lam, dx, dx, z, output_magnitude=True)
+    xc = _dev(x)
+    yc = fc(xc).cpu().numpy()
+    ym = fm(xc).cpu().numpy()
+    np.testing.assert_allclose(ym, np.abs(yc), rtol=1e-4, atol=1e-4)
+    p = O.PathParams(H, W, 4, 4, 0, 1, "fresnel", lam, dx, dx, z)
+    ref = O.fresnel(torch.as_tensor(x.astype(np.complex128)), p, "f64", output_magnitude=False).numpy()
+    assert rel_l2(yc, ref) < 2e-6
+    assert rel_l2(ym, np.abs(ref)) < 2e-6
+    # real input and z-only update
+    xr = np.random.default_rng(0).normal(size=(2, 64, 64)).astype(np.float32)
+    f = hfx.Fresnel(2, 64, 64, lam, 12e-6, 12e-6, 0.2, in_real=True, output_magnitude=True)
+    f.update_z(0.45)
+    p = O.PathParams(64, 64, 4, 4, 0, 1, "fresnel", lam, 12e-6, 12e-6, 0.45)
+    got = f(_dev(xr)).cpu().numpy()
+    assert rel_l2(got, O.fresnel(torch.as_tensor(xr.astype(np.float64)), p, "f64").numpy()) < 2e-6
+
+
+def test_asm_node_reference_kats(gpu):
+    from holoflow_amd import hfx
+
+    # angular_spectrum_test.cu:146-252, z = 0
+    a = hfx.AngularSpectrum(1, 2, 2, 852e-9, 20e-6, 20e-6, 0.0, in_real=True)
+    got = a(_dev(np.array([[1.0, 2.0], [3.0, 4.0]], np.float32))).cpu().numpy()
+    np.testing.assert_allclose(got[0], [[4, 8], [12, 16]], atol=1e-3)
+    a = hfx.AngularSpectrum(1, 2, 2, 852e-9, 20e-6, 20e-6, 0.0, in_real=True, pad=(4, 4))
+    got = a(_dev(np.array([[1.0, 2.0], [3.0, 4.0]], np.float32))).cpu().numpy()
+    np.testing.assert_allclose(got[0], [[0, 0, 0, 0], [0, 16, 32, 0], [0, 48, 64, 0], [0, 0, 0, 0]], atol=1e-3)
+    x = np.array([(k + 1) * (1 + 1j) for k in range(8)], np.complex64).reshape(2, 2, 2)
+    a = hfx.AngularSpectrum(2, 2, 2, 852e-9, 20e-6, 20e-6, 0.0, pad=(4, 4))
+    got = a(_dev(x)).cpu().numpy()
+    for b in range(2):
+        np.testing.assert_allclose(got[b, 1:3, 1:3], 16 * x[b], atol=1e-3)
+        got[b, 1:3, 1:3] = 0
+    np.testing.assert_allclose(got, 0, atol=1e-3)
+    d = np.zeros((1, 4, 4), np.complex64)
+    d[0, 1, 1] = 1.0
+    u = hfx.AngularSpectrum(1, 4, 4, 852e-9, 20e-6, 20e-6, 0.01)(_dev(d)).cpu().numpy()
+    f = hfx.AngularSpectrum(1, 4, 4, 852e-9, 20e-6, 20e-6, 0.01, filt=(0, 100, 0, 0))(_dev(d)).cpu().numpy()
+    np.testing.assert_allclose(f, u, atol=1e-3)
+
+
+@pytest.mark.parametrize("H,W,pad,filt", [(64, 64, (0, 0), None), (48, 64, (64, 96), None),
+                                          (96, 128, (0, 0), (10, 40, 3, 5))])
+def test_asm_node_vs_oracle(gpu, H, W, pad, filt):
+    from holoflow_amd import hfx
+
+    rng = np.random.default_rng(H + W)
+    x = (rng.normal(size=(3, H, W)) + 1j * rng.normal(size=(3, H, W))).astype(np.complex64)
+    a = hfx.AngularSpectrum(3, H, W, 852e-9, 12e-6, 12e-6, 0.3, pad=pad, filt=filt)
+    got = a(_dev(x)).cpu().numpy()
+    p = O.PathParams(H, W, 4, 4, 0, 1, "asm", 852e-9, 12e-6, 12e-6, 0.3, pad=pad, filt=filt)
+    ref = O.angular_spectrum(torch.as_tensor(x.astype(np.complex128)), p, "f64").numpy()
+    assert rel_l2(got, ref) < 2e-6
+
+
+def test_small_ops_kats(gpu):
+    from holoflow_amd import hfx
+
+    # fftshift_test.cu:105-121 and the odd-length rule
+    x = torch.arange(1, 13, dtype=torch.float32).reshape(3, 4).cuda()
+    np.testing.assert_array_equal(hfx.fftshift(x, [1]).cpu().numpy(), [[3, 4, 1, 2], [7, 8, 5, 6], [11, 12, 9, 10]])
+    np.testing.assert_array_equal(hfx.fftshift(x, [0, 1]).cpu().numpy(),
+                                  O.fftshift_ref(x.cpu(), (0, 1)).numpy())
+    # mean_test.cu:109-141 (mean over the plane axis)
+    m = hfx.weighted_mean(torch.tensor([[3.0, 8.0], [1.0, 5.0], [2.0, 7.0]]).cuda())
+    np.testing.assert_allclose(m.cpu().numpy(), [2.0, 20.0 / 3.0], rtol=1e-6)
+    # abs-mean of complex planes with weights
+    z = torch.tensor([[3 + 4j, 0j], [6 + 8j, 1j]], dtype=torch.complex64).cuda()
+    w = torch.tensor([2.0, 0.5]).cuda()
+    np.testing.assert_allclose(hfx.weighted_mean(z, w).cpu().numpy(), [(2 * 5 + 0.5 * 10) / 2, 0.25], rtol=1e-6)
+    # conversion: all 4096 12-bit codes are exact
+    codes = np.arange(4096, dtype=np.uint16)
+    np.testing.assert_array_equal(hfx.convert_u16(_dev(codes)).cpu().numpy(), codes.astype(np.float32))
+    cc = hfx.convert_u16(_dev(codes), to_complex=True).cpu().numpy()
+    np.testing.assert_array_equal(cc.real, codes) and np.testing.assert_array_equal(cc.imag, 0)
+    # fft_test.cu:86 / rfft_test.cu:86 along axis 1
+    f = hfx.dft_axis(torch.tensor([[1.0, 2, 3, 4], [0, -1, 2, -3]]).cuda(), 1).cpu().numpy()
+    np.testing.assert_allclose(f, [[10, -2 + 2j, -2, -2 - 2j], [-2, -2 - 2j, 6, -2 + 2j]], atol=1e-5)
+    xr = torch.tensor([[1.0, 2, 3, 4, 5, 6], [0, -1, 2, -3, 1, 4]])
+    r = hfx.dft_axis(xr.cuda(), 1, 0, 4).cpu().numpy()
+    np.testing.assert_allclose(r, np.fft.rfft(xr.numpy().astype(np.float64), axis=1), rtol=1e-4, atol=1e-5)
