@@ -82,6 +82,7 @@ struct hfx_plan {
   int H = 0, W = 0, OH = 0, OW = 0;
   long long npix = 0, onpix = 0;
   int nb = 0, nbc = 0, n_planes = 0, n_out = 3, blk_c = 1, wm_rows = 0;
+  bool fused_row = false;  // time DFT fused with the forward row FFT
   DevArena arena;
   float2 *planes_nat = nullptr, *planes_blk = nullptr;
   float2 *tw_w = nullptr, *tw_h = nullptr;
@@ -201,7 +202,9 @@ hfx_status hfx_plan_create(const hfx_plan_desc *dp, hfx_stream_t stream, hfx_pla
 
   hipStream_t st = S(stream);
   DevArena &A = p->arena;
-  HFX_TRY(A.alloc(&p->planes_nat, (size_t)p->n_planes * p->npix));
+  p->fused_row = p->OH == p->H && p->OW == p->W && hfx::dft_row_supported(p->W, p->nbc, d.window);
+  // natural-layout planes: only the stand-alone time DFT needs them
+  HFX_TRY(A.alloc(&p->planes_nat, p->fused_row ? 1 : (size_t)p->n_planes * p->npix));
   HFX_TRY(A.alloc(&p->planes_blk, (size_t)p->n_planes * p->onpix));
   HFX_TRY(A.alloc(&p->tw_w, p->OW));
   HFX_TRY(A.alloc(&p->tw_h, p->OH));
@@ -265,8 +268,8 @@ static hfx_status check_ring(const hfx_plan *p, const uint16_t *ring, int64_t ri
   return HFX_OK;
 }
 
-static hfx_status run_dft(hfx_plan *p, const uint16_t *ring, int64_t ring_frames, int64_t first,
-                          float2 *planes, hipStream_t st) {
+static hfx::DftLaunch dft_launch(hfx_plan *p, const uint16_t *ring, int64_t ring_frames,
+                                 int64_t first, float2 *planes) {
   hfx::DftLaunch L{};
   L.ring = ring;
   L.ring_frames = ring_frames;
@@ -286,7 +289,12 @@ static hfx_status run_dft(hfx_plan *p, const uint16_t *ring, int64_t ring_frames
   L.wm_f16hi = reinterpret_cast<const uint16_t *>(p->wm16hi);
   L.wm_f16lo = reinterpret_cast<const uint16_t *>(p->wm16lo);
   L.wm_bf16 = reinterpret_cast<const uint16_t *>(p->wmbf);
-  int rc = hfx::launch_time_dft(L, st);
+  return L;
+}
+
+static hfx_status run_dft(hfx_plan *p, const uint16_t *ring, int64_t ring_frames, int64_t first,
+                          float2 *planes, hipStream_t st) {
+  int rc = hfx::launch_time_dft(dft_launch(p, ring, ring_frames, first, planes), st);
   if (rc != 0) return fail(HFX_E_HIP, "time DFT launch failed (" + std::to_string(rc) + ")");
   return HFX_OK;
 }
@@ -314,7 +322,13 @@ static hfx::PassArgs base_pass(int H, int W, int planes, int blk_c) {
 
 static hfx_status window_moments(hfx_plan *p, const uint16_t *ring, int64_t ring_frames,
                                  int64_t first, float *out, hipStream_t st) {
-  HFX_TRY(run_dft(p, ring, ring_frames, first, p->planes_nat, st));
+  if (p->fused_row) {
+    hfx::DftLaunch L = dft_launch(p, ring, ring_frames, first, p->planes_blk);
+    if (hfx::launch_time_dft_row(L, p->OH, p->blk_c, p->n_planes, p->tw_w, st) != 0)
+      return fail(HFX_E_HIP, "fused time-DFT/row-FFT launch failed");
+  } else {
+    HFX_TRY(run_dft(p, ring, ring_frames, first, p->planes_nat, st));
+  }
   const long long on = p->onpix;
   hfx::MomentArgs mom{};
   for (int i = 0; i < p->n_out; ++i) mom.out[i] = out + i * on;
@@ -324,18 +338,20 @@ static hfx_status window_moments(hfx_plan *p, const uint16_t *ring, int64_t ring
   mom.power2 = (p->d.flags & HFX_FLAG_POWER2) ? 1 : 0;
   mom.shift = (p->d.flags & HFX_FLAG_FFTSHIFT) ? 1 : 0;
 
-  // row pass: natural planes (lens already applied for Fresnel) -> blocked spectrum rows
-  hfx::PassArgs r = base_pass(p->OH, p->OW, p->n_planes, p->blk_c);
-  r.in = p->planes_nat;
-  r.in_mode = hfx::IN_NAT_C;
-  r.in_h = p->H;
-  r.in_w = p->W;
-  r.off_y = (p->OH - p->H) / 2;
-  r.off_x = (p->OW - p->W) / 2;
-  r.out = p->planes_blk;
-  r.out_mode = hfx::OUT_BLK;
-  r.tw = p->tw_w;
-  if (hfx::launch_row_pass(r, st) != 0) return fail(HFX_E_HIP, "row pass launch failed");
+  if (!p->fused_row) {
+    // row pass: natural planes (lens already applied for Fresnel) -> blocked spectrum rows
+    hfx::PassArgs r = base_pass(p->OH, p->OW, p->n_planes, p->blk_c);
+    r.in = p->planes_nat;
+    r.in_mode = hfx::IN_NAT_C;
+    r.in_h = p->H;
+    r.in_w = p->W;
+    r.off_y = (p->OH - p->H) / 2;
+    r.off_x = (p->OW - p->W) / 2;
+    r.out = p->planes_blk;
+    r.out_mode = hfx::OUT_BLK;
+    r.tw = p->tw_w;
+    if (hfx::launch_row_pass(r, st) != 0) return fail(HFX_E_HIP, "row pass launch failed");
+  }
 
   hfx::PassArgs c = base_pass(p->OH, p->OW, p->n_planes, p->blk_c);
   c.in = p->planes_blk;

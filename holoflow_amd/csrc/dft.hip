@@ -1,271 +1,241 @@
-// Band-limited time DFT of a uint16 frame window, run on the MFMA units of gfx950.
+// Time DFT of a uint16 frame window on MFMA (see dft_core.hh), in two forms:
 //
-// For every pixel p the path needs S_k(p) = sum_n x[n](p) * exp(-2*pi*i*k*n/T) for the band
-// bins k in [kb, kb+nbc) only (rfft.cu:148-171 computes all T/2+1 bins and slices them
-// afterwards, graph_builder.cc:259-266).  Written as a contraction it is
-//     D[2*nbc x P] = Wm[2*nbc x T] * X[T x P],   Wm rows = (cos, -sin) per bin,
-// i.e. a skinny GEMM with K = T.  One wave owns 64 pixels, split into four 16-pixel
-// sub-tiles {base + 4*j + t : j < 16} so that each lane loads 4 adjacent pixels of one
-// frame with a single 8-byte load; the four values feed the B operand of four MFMAs.
+//   k_dft          : stand-alone; writes the band planes [planes][H][W] in natural layout
+//                    (optionally times the Fresnel lens; negative-band conjugate planes for
+//                    the two-sided FFT path).  Used for the per-stage entry point and for
+//                    widths the fused kernel does not cover.
+//   k_dft_row      : fused time DFT + forward row FFT.  A persistent workgroup walks a
+//                    set of image rows; for each row its waves (64 or 128 pixels each) keep
+//                    the band spectra in MFMA registers, push them through LDS into one
+//                    padded row per plane, run one wave-level row FFT per plane and write
+//                    the result once, column-blocked, for the column pass.  The next row's
+//                    first frames are loaded while the current row is transformed.  The
+//                    band planes never make an HBM round trip in natural layout.
 //
-// The samples are centred, x - 2048, before the products.  For 12-bit data this is exact
-// in FP32 and FP16 (|x - 2048| <= 2048 fits the 11-bit FP16 significand), and it changes
-// no bin k != 0 (sum_n exp(-2*pi*i*k*n/T) = 0); bin 0 gets 2048*T added back.
-//
-// Precisions (accumulation is FP32 in all of them):
-//   FP32       : v_mfma_f32_16x16x4_f32, operands FP32.
-//   FP16       : v_mfma_f32_16x16x32_f16, samples exact, twiddles rounded to FP16.
-//   BF16       : v_mfma_f32_16x16x32_bf16, samples and twiddles rounded to BF16.
-//   FP16_SPLIT : two FP16 MFMAs per tile, twiddle = hi + lo (both FP16), samples exact.
-//
-// Epilogue: optional Fresnel lens multiply (separable chirp lx[x]*ly[y]) and, for the
-// two-sided FFT path, the conjugate planes of the negative band.
+// Plane q of the output: q < nb -> bin kb+q; q >= nb (two-sided path) -> conj of computed
+// bin 2*nb - q, i.e. X[T - (kb + 2*nb - q)] of a real input (fft.cu:163-194 +
+// graph_builder.cc:270-287).
+#include "dft_core.hh"
 #include "hfx_kernels.hh"
-
-#include <hip/hip_bf16.h>
-#include <hip/hip_fp16.h>
+#include "wfft.hh"
 
 namespace hfx {
 
-typedef float f4 __attribute__((ext_vector_type(4)));
-typedef _Float16 h8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
-
 struct DftArgs {
-  const uint16_t *ring;  // [ring_frames][npix]
-  long long ring_frames;
-  long long first;       // first frame of the window, in [0, ring_frames)
-  int T;
-  long long npix;
+  DftRing r;
   int W;
-  int kb, nb, nbc;       // band start, planes per sideband, computed bins (nb or nb+1)
-  int both;              // write negative-band conjugate planes at [nb, 2nb)
-  float2 *planes;        // [planes][npix]
-  const float2 *lx, *ly; // Fresnel lens vectors or null
+  int kb, nb, nbc, both;
+  float2 *planes;          // k_dft: natural [planes][npix]; k_dft_row: blocked
+  const float2 *lx, *ly;   // Fresnel lens vectors or null
+  const void *wm_hi, *wm_lo;
+  // k_dft_row only
+  int H, blk_c, n_planes;
+  const float2 *tw;        // W-point FFT twiddles
 };
 
-__device__ __forceinline__ void dft_store(const DftArgs &a, int bin, long long p0, const float *re,
-                                          const float *im) {
-  // bin: computed bin index (kb + bin is the frequency index); p0: first of 4 pixels.
-  float2 v[4];
+// value of computed bin `bin` for 4 pixels, with the centring offset restored on bin 0
+__device__ __forceinline__ void dft_bin_values(const DftArgs &a, const f4 (&acc)[4], int h, int bin,
+                                               float2 (&v)[4]) {
 #pragma unroll
-  for (int t = 0; t < 4; ++t) v[t] = make_float2(re[t], im[t]);
+  for (int t = 0; t < 4; ++t) v[t] = make_float2(acc[t][2 * h], acc[t][2 * h + 1]);
   if (a.kb + bin == 0) {
 #pragma unroll
-    for (int t = 0; t < 4; ++t) v[t].x += 2048.0f * (float)a.T;
+    for (int t = 0; t < 4; ++t) v[t].x += 2048.0f * (float)a.r.T;
   }
-  if (p0 >= a.npix) return;
+}
+
+// ----------------------------------------------------------------------- stand-alone
+template <int NBT, int MODE>
+__global__ __launch_bounds__(256) void k_dft(DftArgs a) {
+  extern __shared__ float4 lds_dyn[];
+  dft_stage_matrix<NBT, MODE>(lds_dyn, a.wm_hi, a.wm_lo, a.r.T, threadIdx.x, 256);
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const long long p0w = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+  uint2 pre[1][dft_pre_steps<MODE>()];
+  dft_prefetch<MODE, 1>(a.r, p0w, lane, pre);
+  f4 acc1[1][NBT][4];
+  dft_tile<NBT, MODE, 1>(a.r, lds_dyn, p0w, lane, pre, acc1);
+  auto &acc = acc1[0];
+
+  const long long p0 = p0w + 4 * (lane & 15);
+  if (p0 >= a.r.npix) return;
   float2 L[4];
-  bool lens = a.lx != nullptr;
+  const bool lens = a.lx != nullptr;
   if (lens) {
-    int y = (int)(p0 / a.W), x = (int)(p0 % a.W);  // W % 4 == 0: 4 pixels share a row
-    float2 ly = a.ly[y];
+    const int y = (int)(p0 / a.W), x = (int)(p0 % a.W);  // W % 4 == 0: same row
+    const float2 ly = a.ly[y];
 #pragma unroll
     for (int t = 0; t < 4; ++t) L[t] = cmul(a.lx[x + t], ly);
   }
-  if (bin < a.nb) {
-    float2 o[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) o[t] = lens ? cmul(v[t], L[t]) : v[t];
-    float4 *dst = reinterpret_cast<float4 *>(a.planes + (long long)bin * a.npix + p0);
-    dst[0] = make_float4(o[0].x, o[0].y, o[1].x, o[1].y);
-    dst[1] = make_float4(o[2].x, o[2].y, o[3].x, o[3].y);
-  }
-  if (a.both) {
-    int j = a.nb - bin;  // negative-band plane holding conj(X[kb+bin]) = X[T-(kb+bin)]
-    if (j >= 0 && j < a.nb) {
-      float2 o[4];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        float2 c = make_float2(v[t].x, -v[t].y);
-        o[t] = lens ? cmul(c, L[t]) : c;
-      }
-      float4 *dst = reinterpret_cast<float4 *>(a.planes + (long long)(a.nb + j) * a.npix + p0);
-      dst[0] = make_float4(o[0].x, o[0].y, o[1].x, o[1].y);
-      dst[1] = make_float4(o[2].x, o[2].y, o[3].x, o[3].y);
-    }
-  }
-}
-
-__device__ __forceinline__ uint2 load4(const DftArgs &a, long long frame, long long p0) {
-  if (p0 >= a.npix) return make_uint2(0x08000800u, 0x08000800u);  // 2048: centred zero
-  const uint16_t *src = a.ring + frame * a.npix + p0;
-  return *reinterpret_cast<const uint2 *>(src);
-}
-
-// ---------------------------------------------------------------- FP32 MFMA 16x16x4
-template <int NBT>
-__global__ __launch_bounds__(256) void k_dft_f32(DftArgs a, const float *wm) {
-  extern __shared__ float lds_w[];  // [NBT*16][T+1]
-  const int T = a.T, TP = T + 1;
-  for (int i = threadIdx.x; i < NBT * 16 * T; i += blockDim.x) {
-    int r = i / T, n = i % T;
-    lds_w[r * TP + n] = wm[i];
-  }
-  __syncthreads();
-
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const long long base = ((long long)blockIdx.x * 4 + wave) * 64;
-  const long long p0 = base + 4 * (lane & 15);
-  const int sub = lane >> 4;  // k index inside the MFMA step / row group of the output
-
-  f4 acc[NBT][4];
-#pragma unroll
-  for (int bt = 0; bt < NBT; ++bt)
-#pragma unroll
-    for (int t = 0; t < 4; ++t) acc[bt][t] = f4{0.f, 0.f, 0.f, 0.f};
-
-  long long frame = a.first + sub;
-  if (frame >= a.ring_frames) frame -= a.ring_frames;
-  for (int s = 0; s < T / 4; ++s) {
-    uint2 raw = load4(a, frame, p0);
-    float x[4];
-    x[0] = (float)(raw.x & 0xffffu) - 2048.0f;
-    x[1] = (float)(raw.x >> 16) - 2048.0f;
-    x[2] = (float)(raw.y & 0xffffu) - 2048.0f;
-    x[3] = (float)(raw.y >> 16) - 2048.0f;
-    const int n = 4 * s + sub;
-#pragma unroll
-    for (int bt = 0; bt < NBT; ++bt) {
-      float w = lds_w[(bt * 16 + (lane & 15)) * TP + n];
-#pragma unroll
-      for (int t = 0; t < 4; ++t)
-        acc[bt][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(w, x[t], acc[bt][t], 0, 0, 0);
-    }
-    frame += 4;
-    if (frame >= a.ring_frames) frame -= a.ring_frames;
-  }
-
-  // D[row][pixel]: row = bt*16 + 4*sub + r, pixel = base + 4*(lane&15) + t
 #pragma unroll
   for (int bt = 0; bt < NBT; ++bt) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      int bin = bt * 8 + 2 * sub + h;
-      if (bin < a.nbc) {
-        float re[4], im[4];
+      const int bin = bt * 8 + 2 * (lane >> 4) + h;
+      if (bin >= a.nbc) continue;
+      float2 v[4];
+      dft_bin_values(a, acc[bt], h, bin, v);
+#pragma unroll
+      for (int side = 0; side < 2; ++side) {
+        int q;
+        if (side == 0) {
+          if (bin >= a.nb) continue;
+          q = bin;
+        } else {
+          if (!a.both) continue;
+          const int j = a.nb - bin;
+          if (j < 0 || j >= a.nb) continue;
+          q = a.nb + j;
+        }
+        float2 o[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-          re[t] = acc[bt][t][2 * h];
-          im[t] = acc[bt][t][2 * h + 1];
+          float2 c = side ? make_float2(v[t].x, -v[t].y) : v[t];
+          o[t] = lens ? cmul(c, L[t]) : c;
         }
-        dft_store(a, bin, p0, re, im);
+        float4 *dst = reinterpret_cast<float4 *>(a.planes + (long long)q * a.r.npix + p0);
+        dst[0] = make_float4(o[0].x, o[0].y, o[1].x, o[1].y);
+        dst[1] = make_float4(o[2].x, o[2].y, o[3].x, o[3].y);
       }
     }
   }
 }
 
-// ---------------------------------------------------------------- 16x16x32 FP16 / BF16
-// MODE 0: FP16, 1: BF16, 2: FP16 hi/lo split twiddles.
-template <int NBT, int MODE>
-__global__ __launch_bounds__(256) void k_dft_lp(DftArgs a, const uint16_t *wm_hi,
-                                                const uint16_t *wm_lo) {
-  // operands stored as raw 16-bit words; row stride padded by 8 elements
-  extern __shared__ uint16_t lds16[];
-  const int T = a.T, TP = T + 8;
-  uint16_t *lhi = lds16;
-  uint16_t *llo = lds16 + NBT * 16 * TP;
-  for (int i = threadIdx.x; i < NBT * 16 * T; i += blockDim.x) {
-    int r = i / T, n = i % T;
-    lhi[r * TP + n] = wm_hi[i];
-    if (MODE == 2) llo[r * TP + n] = wm_lo[i];
-  }
+// ----------------------------------------------------------------------- fused with row FFT
+// Planes per LDS group: the group's padded rows fill ~127 KB (15 planes at W = 1024), which
+// leaves room for the pass twiddles and the operand matrix of the MFMA loop.
+template <int W> constexpr int row_group() { return 130560 / (WaveFft<W>::SCR * 8); }
+
+// Waves per row: one per 64 pixels up to 512 wide, one per 128 pixels (two MFMA tiles
+// sharing every operand load) beyond, which keeps 8 waves and a 256-register budget.
+template <int W> constexpr int row_tpw() { return W > 512 ? 2 : 1; }
+template <int W> constexpr int row_threads() { return W / row_tpw<W>(); }
+
+// MULTI: the planes need more than one LDS group (two-sided bands at large widths); a
+// separate instantiation so the common single-group kernel carries no spill code.
+template <int W, int NBT, int MODE, bool MULTI>
+__global__ __launch_bounds__(row_threads<W>()) void k_dft_row(DftArgs a) {
+  using F = WaveFft<W>;
+  constexpr int TPW = row_tpw<W>();
+  constexpr int NT = row_threads<W>();
+  constexpr int NW = NT / 64;
+  constexpr int E = F::E;
+  constexpr int GB = row_group<W>();
+  __shared__ float2 scr[GB * F::SCR];  // one padded row per plane of the current group
+  __shared__ float2 twl[F::TW];
+  extern __shared__ float4 wmat[];     // operand matrix of the MFMA loop
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  dft_stage_matrix<NBT, MODE>(wmat, a.wm_hi, a.wm_lo, a.r.T, tid, NT);
+  F::build_twiddles(twl, a.tw, tid, NT);
   __syncthreads();
 
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const long long base = ((long long)blockIdx.x * 4 + wave) * 64;
-  const long long p0 = base + 4 * (lane & 15);
-  const int sub = lane >> 4;
+  const long long plane_elems = (long long)a.H * W;
+  const bool lens = a.lx != nullptr;
+  const int G = gridDim.x;
+  int y = xcd_tile(blockIdx.x, G);
+  uint2 pre[TPW][dft_pre_steps<MODE>()];
+  if (y < a.H) dft_prefetch<MODE, TPW>(a.r, (long long)y * W + wave * 64 * TPW, lane, pre);
 
-  f4 acc[NBT][4];
-#pragma unroll
-  for (int bt = 0; bt < NBT; ++bt)
-#pragma unroll
-    for (int t = 0; t < 4; ++t) acc[bt][t] = f4{0.f, 0.f, 0.f, 0.f};
+  for (; y < a.H; y += G) {
+    // An opaque per-row copy of the lane index keeps the many loop-invariant LDS
+    // addresses of the scatter and the transforms out of registers across rows.
+    int lane;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(lane) : "v"(tid & 63));
+    f4 acc[TPW][NBT][4];
+    dft_tile<NBT, MODE, TPW>(a.r, wmat, (long long)y * W + wave * 64 * TPW, lane, pre, acc);
+    // the next row's first frames load while this row is scattered and transformed
+    if (y + G < a.H) dft_prefetch<MODE, TPW>(a.r, (long long)(y + G) * W + wave * 64 * TPW, lane, pre);
 
-  for (int s = 0; s < T / 32; ++s) {
-    // lane holds frames n = 32*s + 8*sub + q, q < 8, for its 4 pixels
-    uint2 raw[8];
+    // scatter planes [q0, q0+GB) from the MFMA registers into padded LDS rows
+    auto scatter = [&](int q0) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      long long frame = a.first + 32 * s + 8 * sub + q;
-      if (frame >= a.ring_frames) frame -= a.ring_frames;
-      raw[q] = load4(a, frame, p0);
-    }
-    h8 xh[4];
-    bf8 xb[4];
+      for (int i = 0; i < TPW; ++i) {
+        const int x0 = (wave * TPW + i) * 64 + 4 * (lane & 15);  // first of the lane's 4 pixels
+        float2 L[4];
+        if (lens) {
+          const float2 ly = a.ly[y];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      float v0 = (float)(raw[q].x & 0xffffu) - 2048.0f;
-      float v1 = (float)(raw[q].x >> 16) - 2048.0f;
-      float v2 = (float)(raw[q].y & 0xffffu) - 2048.0f;
-      float v3 = (float)(raw[q].y >> 16) - 2048.0f;
-      if (MODE == 1) {
-        xb[0][q] = (__bf16)v0;
-        xb[1][q] = (__bf16)v1;
-        xb[2][q] = (__bf16)v2;
-        xb[3][q] = (__bf16)v3;
-      } else {
-        xh[0][q] = (_Float16)v0;
-        xh[1][q] = (_Float16)v1;
-        xh[2][q] = (_Float16)v2;
-        xh[3][q] = (_Float16)v3;
-      }
-    }
-    const int k0 = 32 * s + 8 * sub;
+          for (int t = 0; t < 4; ++t) L[t] = cmul(a.lx[x0 + t], ly);
+        }
 #pragma unroll
-    for (int bt = 0; bt < NBT; ++bt) {
-      const int row = bt * 16 + (lane & 15);
-      const uint4 whi = *reinterpret_cast<const uint4 *>(&lhi[row * TP + k0]);
-      if (MODE == 1) {
-        bf8 w = __builtin_bit_cast(bf8, whi);
+        for (int bt = 0; bt < NBT; ++bt) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
-          acc[bt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, xb[t], acc[bt][t], 0, 0, 0);
-      } else {
-        h8 w = __builtin_bit_cast(h8, whi);
+          for (int h = 0; h < 2; ++h) {
+            const int bin = bt * 8 + 2 * (lane >> 4) + h;
+            if (bin >= a.nbc) continue;
+            float2 v[4];
+            dft_bin_values(a, acc[i][bt], h, bin, v);
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
-          acc[bt][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(w, xh[t], acc[bt][t], 0, 0, 0);
-        if (MODE == 2) {
-          const uint4 wlo = *reinterpret_cast<const uint4 *>(&llo[row * TP + k0]);
-          h8 wl = __builtin_bit_cast(h8, wlo);
+            for (int side = 0; side < 2; ++side) {
+              int q;
+              if (side == 0) {
+                if (bin >= a.nb) continue;
+                q = bin;
+              } else {
+                if (!a.both) continue;
+                const int j = a.nb - bin;
+                if (j < 0 || j >= a.nb) continue;
+                q = a.nb + j;
+              }
+              if (q < q0 || q >= q0 + GB) continue;
+              float2 *dst = scr + (q - q0) * F::SCR;
 #pragma unroll
-          for (int t = 0; t < 4; ++t)
-            acc[bt][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wl, xh[t], acc[bt][t], 0, 0, 0);
+              for (int t = 0; t < 4; ++t) {
+                float2 c = side ? make_float2(v[t].x, -v[t].y) : v[t];
+                dst[F::pad(x0 + t)] = lens ? cmul(c, L[t]) : c;
+              }
+            }
+          }
         }
       }
-    }
-  }
-
+    };
+    // one wave per plane row of the group: forward row FFT, then the column-blocked store
+    auto transform = [&](int q0) {
+      const int q1 = min(q0 + GB, a.n_planes);
+      for (int q = q0 + wave; q < q1; q += NW) {
+        float2 *ws = scr + (q - q0) * F::SCR;
+        float2 v[E];
 #pragma unroll
-  for (int bt = 0; bt < NBT; ++bt) {
+        for (int m = 0; m < E; ++m) v[m] = ws[F::pad(lane + 64 * m)];
+        F::run(v, ws, twl, lane, false);
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      int bin = bt * 8 + 2 * sub + h;
-      if (bin < a.nbc) {
-        float re[4], im[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          re[t] = acc[bt][t][2 * h];
-          im[t] = acc[bt][t][2 * h + 1];
+        for (int m = 0; m < E; ++m) {
+          const int x = lane + 64 * m;
+          a.planes[(long long)q * plane_elems + (long long)(x / a.blk_c) * a.H * a.blk_c +
+                   (long long)y * a.blk_c + (x % a.blk_c)] = v[m];
         }
-        dft_store(a, bin, p0, re, im);
+      }
+    };
+    if constexpr (!MULTI) {
+      // every plane fits one group, so the MFMA accumulators are dead before the
+      // transforms need their registers
+      scatter(0);
+      __syncthreads();
+      transform(0);
+      __syncthreads();
+    } else {
+      for (int q0 = 0; q0 < a.n_planes; q0 += GB) {
+        scatter(q0);
+        __syncthreads();
+        transform(q0);
+        __syncthreads();
       }
     }
   }
 }
 
-template <int NBT>
-static int launch_nbt(const DftLaunch &L, hipStream_t st) {
-  DftArgs a;
-  a.ring = L.ring;
-  a.ring_frames = L.ring_frames;
-  a.first = L.first;
-  a.T = L.T;
-  a.npix = L.npix;
+// ----------------------------------------------------------------------- launch
+static DftArgs make_args(const DftLaunch &L) {
+  DftArgs a{};
+  a.r.ring = L.ring;
+  a.r.ring_frames = L.ring_frames;
+  a.r.first = L.first;
+  a.r.npix = L.npix;
+  a.r.T = L.T;
   a.W = L.W;
   a.kb = L.kb;
   a.nb = L.nb;
@@ -274,37 +244,97 @@ static int launch_nbt(const DftLaunch &L, hipStream_t st) {
   a.planes = L.planes;
   a.lx = L.lx;
   a.ly = L.ly;
-  long long waves = (L.npix + 63) / 64;
-  dim3 grid((unsigned)((waves + 3) / 4));
-  if (L.precision == 0) {
-    size_t lds = (size_t)NBT * 16 * (L.T + 1) * sizeof(float);
-    hipLaunchKernelGGL(k_dft_f32<NBT>, grid, dim3(256), lds, st, a, L.wm_f32);
-  } else {
-    size_t lds = (size_t)NBT * 16 * (L.T + 8) * 2 * (L.precision == 3 ? 2 : 1);
-    const uint16_t *hi = L.precision == 2 ? L.wm_bf16 : L.wm_f16hi;
-    if (L.precision == 1)
-      hipLaunchKernelGGL((k_dft_lp<NBT, 0>), grid, dim3(256), lds, st, a, hi, L.wm_f16lo);
-    else if (L.precision == 2)
-      hipLaunchKernelGGL((k_dft_lp<NBT, 1>), grid, dim3(256), lds, st, a, hi, L.wm_f16lo);
-    else
-      hipLaunchKernelGGL((k_dft_lp<NBT, 2>), grid, dim3(256), lds, st, a, hi, L.wm_f16lo);
+  switch (L.precision) {
+    case 0: a.wm_hi = L.wm_f32; break;
+    case 2: a.wm_hi = L.wm_bf16; break;
+    default: a.wm_hi = L.wm_f16hi; break;
   }
+  a.wm_lo = L.wm_f16lo;
+  return a;
+}
+
+template <int NBT, int MODE> static int launch_plain(const DftArgs &a, hipStream_t st) {
+  const long long waves = (a.r.npix + 63) / 64;
+  const size_t lds = dft_matrix_lds_bytes(NBT, MODE, a.r.T);
+  hipLaunchKernelGGL((k_dft<NBT, MODE>), dim3((unsigned)((waves + 3) / 4)), dim3(256), lds, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_time_dft(const DftLaunch &L, hipStream_t st) {
-  int nbt = (2 * L.nbc + 15) / 16;
-  switch (nbt) {
-    case 1: return launch_nbt<1>(L, st);
-    case 2: return launch_nbt<2>(L, st);
-    case 3: return launch_nbt<3>(L, st);
-    case 4: return launch_nbt<4>(L, st);
-    case 5: return launch_nbt<5>(L, st);
-    case 6: return launch_nbt<6>(L, st);
-    case 7: return launch_nbt<7>(L, st);
-    case 8: return launch_nbt<8>(L, st);
-    default: return -2;
+template <int NBT> static int launch_plain_mode(const DftArgs &a, int mode, hipStream_t st) {
+  switch (mode) {
+    case 0: return launch_plain<NBT, 0>(a, st);
+    case 1: return launch_plain<NBT, 1>(a, st);
+    case 2: return launch_plain<NBT, 2>(a, st);
+    case 3: return launch_plain<NBT, 3>(a, st);
   }
+  return -2;
+}
+
+int launch_time_dft(const DftLaunch &L, hipStream_t st) {
+  const DftArgs a = make_args(L);
+  switch ((2 * L.nbc + 15) / 16) {
+    case 1: return launch_plain_mode<1>(a, L.precision, st);
+    case 2: return launch_plain_mode<2>(a, L.precision, st);
+    case 3: return launch_plain_mode<3>(a, L.precision, st);
+    case 4: return launch_plain_mode<4>(a, L.precision, st);
+    case 5: return launch_plain_mode<5>(a, L.precision, st);
+    case 6: return launch_plain_mode<6>(a, L.precision, st);
+    case 7: return launch_plain_mode<7>(a, L.precision, st);
+    case 8: return launch_plain_mode<8>(a, L.precision, st);
+  }
+  return -2;
+}
+
+bool dft_row_supported(int W, int nbc, int T) {
+  return (W == 512 || W == 768 || W == 1024) && (2 * nbc + 15) / 16 <= 2 && T <= 128;
+}
+
+template <int W, int NBT, int MODE> static int launch_row(const DftArgs &a, hipStream_t st) {
+  const size_t lds = dft_matrix_lds_bytes(NBT, MODE, a.r.T);
+  // persistent grid: one workgroup per CU (the LDS footprint allows no more), each
+  // walking rows blockIdx, blockIdx + grid, ...
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const unsigned grid = (unsigned)(a.H < cus ? a.H : cus);
+  if (a.n_planes <= row_group<W>())
+    hipLaunchKernelGGL((k_dft_row<W, NBT, MODE, false>), dim3(grid), dim3(row_threads<W>()), lds, st, a);
+  else
+    hipLaunchKernelGGL((k_dft_row<W, NBT, MODE, true>), dim3(grid), dim3(row_threads<W>()), lds, st, a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int W, int NBT> static int launch_row_mode(const DftArgs &a, int mode, hipStream_t st) {
+  switch (mode) {
+    case 0: return launch_row<W, NBT, 0>(a, st);
+    case 1: return launch_row<W, NBT, 1>(a, st);
+    case 2: return launch_row<W, NBT, 2>(a, st);
+    case 3: return launch_row<W, NBT, 3>(a, st);
+  }
+  return -2;
+}
+
+template <int W> static int launch_row_w(const DftArgs &a, int nbt, int mode, hipStream_t st) {
+  if (nbt == 1) return launch_row_mode<W, 1>(a, mode, st);
+  if (nbt == 2) return launch_row_mode<W, 2>(a, mode, st);
+  return -2;
+}
+
+int launch_time_dft_row(const DftLaunch &L, int H, int blk_c, int n_planes, const float2 *tw,
+                        hipStream_t st) {
+  if (!dft_row_supported(L.W, L.nbc, L.T)) return -2;
+  DftArgs a = make_args(L);
+  a.H = H;
+  a.blk_c = blk_c;
+  a.n_planes = n_planes;
+  a.tw = tw;
+  const int nbt = (2 * L.nbc + 15) / 16;
+  switch (L.W) {
+    case 512: return launch_row_w<512>(a, nbt, L.precision, st);
+    case 768: return launch_row_w<768>(a, nbt, L.precision, st);
+    case 1024: return launch_row_w<1024>(a, nbt, L.precision, st);
+  }
+  return -2;
 }
 
 }  // namespace hfx

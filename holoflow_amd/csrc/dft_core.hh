@@ -1,0 +1,256 @@
+// MFMA building block of the band-limited time DFT (shared by the stand-alone time-DFT
+// kernel and the fused time-DFT + row-FFT kernel).
+//
+// For 64 pixels owned by one wave it computes D[2*nbc x 64] = Wm[2*nbc x T] * X[T x 64],
+// the rows of Wm being (cos, -sin)(2*pi*(kb+b)*n/T) of the band bins b (rfft.cu:148-171
+// computes all T/2+1 bins and graph_builder.cc:259-266 slices the band afterwards).  The
+// 64 pixels are split into four 16-pixel sub-tiles {p0 + 4*j + t : j < 16} so a lane loads
+// 4 adjacent pixels of one frame with a single 8-byte load; the four values feed the B
+// operand of four MFMAs.
+//
+// Samples are centred, x - 2048, before the products.  For 12-bit data this is exact in
+// FP32 and FP16 (|x - 2048| <= 2048 fits the 11-bit FP16 significand) and changes no bin
+// k != 0 (sum_n exp(-2*pi*i*k*n/T) = 0); bin 0 gets 2048*T added back by the caller.
+//
+// MODE 0 FP32      : v_mfma_f32_16x16x4_f32, FP32 operands
+// MODE 1 FP16      : v_mfma_f32_16x16x32_f16, exact samples, twiddles rounded to FP16
+// MODE 2 BF16      : v_mfma_f32_16x16x32_bf16, samples and twiddles rounded to BF16
+// MODE 3 FP16 split: two FP16 MFMAs per tile with twiddle = hi + lo, exact samples
+// Accumulation is FP32 in every mode.
+//
+// Output layout (MFMA D registers): acc[bt][t][r] holds row bt*16 + 4*(lane>>4) + r for
+// pixel p0 + 4*(lane&15) + t; rows 2b / 2b+1 are Re / Im of bin b, so a lane holds bins
+// bt*8 + 2*(lane>>4) + {0, 1}.
+#pragma once
+
+#include "hfx_device.hh"
+
+namespace hfx {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+struct DftRing {
+  const uint16_t *ring;   // [ring_frames][npix]
+  long long ring_frames;
+  long long first;        // first frame of the window, in [0, ring_frames)
+  long long npix;
+  int T;
+};
+
+__device__ __forceinline__ uint2 dft_load4(const DftRing &r, long long frame, long long p) {
+  if (p >= r.npix) return make_uint2(0x08000800u, 0x08000800u);  // 2048: a centred zero
+  return *reinterpret_cast<const uint2 *>(r.ring + frame * r.npix + p);
+}
+
+// Stages the operand matrix into LDS.  FP32 rows (MODE 0) use a pitch of T+4 floats, so the
+// 16 rows x 4 frame offsets read by one MFMA step fall in 64 distinct banks; 16-bit rows
+// (MODE 1..3) use T+8 elements (16-byte aligned row starts; `lo` rows follow the `hi`
+// rows for MODE 3).
+template <int NBT, int MODE>
+__device__ __forceinline__ void dft_stage_matrix(void *lds, const void *wm_hi, const void *wm_lo,
+                                                 int T, int tid, int nthreads) {
+  if (MODE == 0) {
+    float *l = reinterpret_cast<float *>(lds);
+    const float *w = reinterpret_cast<const float *>(wm_hi);
+    for (int i = tid; i < NBT * 16 * T; i += nthreads) l[(i / T) * (T + 4) + i % T] = w[i];
+  } else {
+    uint16_t *l = reinterpret_cast<uint16_t *>(lds);
+    const uint16_t *hi = reinterpret_cast<const uint16_t *>(wm_hi);
+    const uint16_t *lo = reinterpret_cast<const uint16_t *>(wm_lo);
+    const int TP = T + 8;
+    for (int i = tid; i < NBT * 16 * T; i += nthreads) {
+      l[(i / T) * TP + i % T] = hi[i];
+      if (MODE == 3) l[NBT * 16 * TP + (i / T) * TP + i % T] = lo[i];
+    }
+  }
+}
+
+__host__ __device__ constexpr size_t dft_matrix_lds_bytes(int nbt, int mode, int T) {
+  return mode == 0 ? (size_t)nbt * 16 * (T + 4) * 4 : (size_t)nbt * 16 * (T + 8) * 2 * (mode == 3 ? 2 : 1);
+}
+
+// Loads a wave issues ahead of its MFMA loop: the first 32 frames (MODE 0: 8 steps of 4
+// frames; MODE 1..3: one step of 32), for each of its TPW tiles.
+template <int MODE> constexpr int dft_pre_steps() { return 8; }
+
+__device__ __forceinline__ long long ring_frame(const DftRing &r, long long n) {
+  long long f = r.first + n;
+  return f >= r.ring_frames ? f - r.ring_frames : f;  // n < T <= ring_frames
+}
+
+// The wave owns TPW adjacent 64-pixel tiles starting at p0w (a multiple of 4); tile i
+// covers p0w + 64*i + {4*j + t}.  npix % 4 == 0.
+template <int MODE, int TPW>
+__device__ __forceinline__ void dft_prefetch(const DftRing &r, long long p0w, int lane,
+                                             uint2 (&pre)[TPW][dft_pre_steps<MODE>()]) {
+  const int sub = lane >> 4;
+  const long long p0 = p0w + 4 * (lane & 15);
+  if (MODE == 0) {
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      if (4 * s < r.T) {
+        const long long f = ring_frame(r, 4 * s + sub);
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) pre[i][s] = dft_load4(r, f, p0 + 64 * i);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const long long f = ring_frame(r, 8 * sub + q);
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) pre[i][q] = dft_load4(r, f, p0 + 64 * i);
+    }
+  }
+}
+
+__device__ __forceinline__ void unpack4(uint2 raw, float (&x)[4]) {
+  x[0] = (float)(raw.x & 0xffffu) - 2048.0f;
+  x[1] = (float)(raw.x >> 16) - 2048.0f;
+  x[2] = (float)(raw.y & 0xffffu) - 2048.0f;
+  x[3] = (float)(raw.y >> 16) - 2048.0f;
+}
+
+// MFMA loop over the window, starting from the prefetched first 32 frames `pre` and
+// loading the rest one batch ahead of its use.
+template <int NBT, int MODE, int TPW>
+__device__ __forceinline__ void dft_tile(const DftRing &r, const void *lds, long long p0w, int lane,
+                                         const uint2 (&pre)[TPW][dft_pre_steps<MODE>()],
+                                         f4 (&acc)[TPW][NBT][4]) {
+  const int T = r.T;
+  const int sub = lane >> 4;
+  const long long p0 = p0w + 4 * (lane & 15);
+#pragma unroll
+  for (int i = 0; i < TPW; ++i)
+#pragma unroll
+    for (int bt = 0; bt < NBT; ++bt)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[i][bt][t] = f4{0.f, 0.f, 0.f, 0.f};
+
+  if (MODE == 0) {
+    const float *lw = reinterpret_cast<const float *>(lds);
+    const int TP = T + 4;
+    auto step = [&](const uint2 (&raw)[TPW], int s) {
+      const int n = 4 * s + sub;
+      float x[TPW][4];
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) unpack4(raw[i], x[i]);
+#pragma unroll
+      for (int bt = 0; bt < NBT; ++bt) {
+        const float w = lw[(bt * 16 + (lane & 15)) * TP + n];
+#pragma unroll
+        for (int i = 0; i < TPW; ++i)
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+            acc[i][bt][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(w, x[i][t], acc[i][bt][t], 0, 0, 0);
+      }
+    };
+    const int steps = T / 4;
+    // steps beyond the prefetched 8 come in batches of K, one batch ahead
+    constexpr int K = 8 / TPW;
+    uint2 nxt[K][TPW];
+    auto load_batch = [&](int s0) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if (s0 + k < steps) {
+          const long long f = ring_frame(r, 4 * (s0 + k) + sub);
+#pragma unroll
+          for (int i = 0; i < TPW; ++i) nxt[k][i] = dft_load4(r, f, p0 + 64 * i);
+        }
+      }
+    };
+    if (steps > 8) load_batch(8);
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      if (s < steps) {
+        uint2 raw[TPW];
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) raw[i] = pre[i][s];
+        step(raw, s);
+      }
+    }
+    for (int s0 = 8; s0 < steps; s0 += K) {
+      uint2 cur[K][TPW];
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) cur[k][i] = nxt[k][i];
+      if (s0 + K < steps) load_batch(s0 + K);
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (s0 + k < steps) step(cur[k], s0 + k);
+    }
+  } else {
+    const uint16_t *lhi = reinterpret_cast<const uint16_t *>(lds);
+    const int TP = T + 8;
+    const uint16_t *llo = lhi + NBT * 16 * TP;
+    uint2 nxt[TPW][8];
+    auto load_step = [&](int s) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const long long f = ring_frame(r, 32 * s + 8 * sub + q);
+#pragma unroll
+        for (int i = 0; i < TPW; ++i) nxt[i][q] = dft_load4(r, f, p0 + 64 * i);
+      }
+    };
+    for (int s = 0; s < T / 32; ++s) {
+      uint2 raw[TPW][8];
+#pragma unroll
+      for (int i = 0; i < TPW; ++i)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) raw[i][q] = s == 0 ? pre[i][q] : nxt[i][q];
+      if (s + 1 < T / 32) load_step(s + 1);  // next step's frames in flight
+      h8 xh[TPW][4];
+      bf8 xb[TPW][4];
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          float x[4];
+          unpack4(raw[i][q], x);
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            if (MODE == 2)
+              xb[i][t][q] = (__bf16)x[t];
+            else
+              xh[i][t][q] = (_Float16)x[t];
+          }
+        }
+      }
+      const int k0 = 32 * s + 8 * sub;
+#pragma unroll
+      for (int bt = 0; bt < NBT; ++bt) {
+        const int row = bt * 16 + (lane & 15);
+        const uint4 whi = *reinterpret_cast<const uint4 *>(&lhi[row * TP + k0]);
+        if (MODE == 2) {
+          bf8 w = __builtin_bit_cast(bf8, whi);
+#pragma unroll
+          for (int i = 0; i < TPW; ++i)
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+              acc[i][bt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, xb[i][t], acc[i][bt][t], 0, 0, 0);
+        } else {
+          h8 w = __builtin_bit_cast(h8, whi);
+#pragma unroll
+          for (int i = 0; i < TPW; ++i)
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+              acc[i][bt][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(w, xh[i][t], acc[i][bt][t], 0, 0, 0);
+          if (MODE == 3) {
+            const uint4 wlo = *reinterpret_cast<const uint4 *>(&llo[row * TP + k0]);
+            h8 wl = __builtin_bit_cast(h8, wlo);
+#pragma unroll
+            for (int i = 0; i < TPW; ++i)
+#pragma unroll
+              for (int t = 0; t < 4; ++t)
+                acc[i][bt][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wl, xh[i][t], acc[i][bt][t], 0, 0, 0);
+          }
+        }
+      }
+    }
+  }
+}
+
+}  // namespace hfx

@@ -44,8 +44,9 @@ constexpr int pass_threads(int tile) {
   t = (t + 63) / 64 * 64;
   return t < 64 ? 64 : (t > 1024 ? 1024 : t);
 }
-// LDS columns of a column-pass tile: 8 (64-byte runs) up to 1024 rows, 4 beyond.
-constexpr int col_lds_width(int n) { return n > 1024 ? 4 : 8; }
+// LDS columns of a column-pass tile.  Four columns give W/4 tiles per plane (256 for a
+// 1024-wide plane, one per CU) while each tile is still one contiguous run of H*4*8 bytes.
+constexpr int col_lds_width(int) { return 4; }
 
 template <int N, bool COLS> struct Geo {
   static constexpr int CL = col_lds_width(N);
@@ -94,7 +95,7 @@ __global__ __launch_bounds__((Geo<N, COLS>::NT)) void k_pass(PassArgs a) {
   for (int i = tid; i < N; i += G::NT) twl[i] = a.tw[i];
 
   const int C = a.blk_c;
-  const int y0 = COLS ? 0 : blockIdx.x * G::ROWS;
+  const int y0 = COLS ? 0 : xcd_tile(blockIdx.x, gridDim.x) * G::ROWS;
   const int x0 = COLS ? blockIdx.x * C : 0;
   const int P = MOM ? a.planes : 1;
   const long long pbase = MOM ? 0 : blockIdx.y;
@@ -109,6 +110,39 @@ __global__ __launch_bounds__((Geo<N, COLS>::NT)) void k_pass(PassArgs a) {
       for (int it = 0; it < G::E; ++it) acc[k][it] = 0.f;
   }
 
+  // Global loads of one plane's tile into registers; the next plane's loads are issued
+  // before the current plane's transform so their latency hides behind the LDS work.
+  float2 pre[G::E];
+  auto load_plane = [&](long long p, int t) {
+#pragma unroll
+    for (int it = 0; it < G::E; ++it) {
+      int e = t + it * G::NT;
+      pre[it] = make_float2(0.f, 0.f);
+      if (e < G::TILE) {
+        int li, y, x;
+        tile_elem<N, COLS>(e, KIND != NAT_TO_BLK, C, y0, x0, li, y, x);
+        if (y < a.H && x >= 0) {
+          if (KIND == NAT_TO_BLK) {
+            const int yi = y - a.off_y, xi = x - a.off_x;
+            if (yi >= 0 && yi < a.in_h && xi >= 0 && xi < a.in_w) {
+              const long long o = p * (long long)a.in_h * a.in_w + (long long)yi * a.in_w + xi;
+              float2 v;
+              if (a.in_mode == IN_NAT_C)
+                v = reinterpret_cast<const float2 *>(a.in)[o];
+              else
+                v = make_float2(reinterpret_cast<const float *>(a.in)[o], 0.f);
+              if (a.pre_lx) v = cmul(v, cmul(a.pre_lx[xi], a.pre_ly[yi]));
+              pre[it] = v;
+            }
+          } else {
+            pre[it] = reinterpret_cast<const float2 *>(a.in)[p * plane_elems + blk_index(y, x, a.H, C)];
+          }
+        }
+      }
+    }
+  };
+  load_plane(pbase, tid);
+
   for (int pp = 0; pp < P; ++pp) {
     const long long p = pbase + pp;
     // An opaque copy of the thread index per plane keeps the per-element index arithmetic
@@ -116,33 +150,18 @@ __global__ __launch_bounds__((Geo<N, COLS>::NT)) void k_pass(PassArgs a) {
     int t;
     asm volatile("v_mov_b32 %0, %1" : "=v"(t) : "v"(tid));
 
-    // ---------------------------------------------------------------- load
+    // ---------------------------------------------------------------- registers -> LDS
 #pragma unroll
     for (int it = 0; it < G::E; ++it) {
       int e = t + it * G::NT;
       if (e < G::TILE) {
         int li, y, x;
         tile_elem<N, COLS>(e, KIND != NAT_TO_BLK, C, y0, x0, li, y, x);
-        float2 v = make_float2(0.f, 0.f);
-        if (y < a.H && x >= 0) {
-          if (KIND == NAT_TO_BLK) {
-            const int yi = y - a.off_y, xi = x - a.off_x;
-            if (yi >= 0 && yi < a.in_h && xi >= 0 && xi < a.in_w) {
-              const long long o = p * (long long)a.in_h * a.in_w + (long long)yi * a.in_w + xi;
-              if (a.in_mode == IN_NAT_C)
-                v = reinterpret_cast<const float2 *>(a.in)[o];
-              else
-                v = make_float2(reinterpret_cast<const float *>(a.in)[o], 0.f);
-              if (a.pre_lx) v = cmul(v, cmul(a.pre_lx[xi], a.pre_ly[yi]));
-            }
-          } else {
-            v = reinterpret_cast<const float2 *>(a.in)[p * plane_elems + blk_index(y, x, a.H, C)];
-          }
-        }
-        lds[li] = v;
+        lds[li] = pre[it];
       }
     }
     __syncthreads();
+    if (pp + 1 < P) load_plane(p + 1, t);
 
     // ---------------------------------------------------------------- transform(s)
     if (KIND == BLK_MID) {
@@ -242,14 +261,13 @@ bool row_pass_supported(int W) {
 }
 
 int pick_blk_c(int W, int H) {
-  const int cl = H > 1024 ? 4 : 8;
-  for (int c = cl; c > 1; c /= 2)
+  for (int c = col_lds_width(H); c > 1; c /= 2)
     if (W % c == 0) return c;
   return 1;
 }
 
 bool col_pass_supported(int H, int blk_c) {
-  return row_pass_supported(H) && blk_c >= 1 && blk_c <= (H > 1024 ? 4 : 8);
+  return row_pass_supported(H) && blk_c >= 1 && blk_c <= col_lds_width(H);
 }
 
 template <int N, bool COLS, int KIND> static int launch_n(const PassArgs &a, hipStream_t st) {
@@ -268,6 +286,7 @@ template <bool COLS, int KIND> static int dispatch(const PassArgs &a, int n, hip
 }
 
 int launch_row_pass(const PassArgs &a, hipStream_t st) {
+  if (wave_pass_supported(a.W)) return launch_wave_row_pass(a, st);
   if (a.in_mode != IN_BLK) {
     if (a.out_mode != OUT_BLK) return -4;
     return dispatch<false, NAT_TO_BLK>(a, a.W, st);
@@ -279,6 +298,7 @@ int launch_row_pass(const PassArgs &a, hipStream_t st) {
 
 int launch_col_pass(const PassArgs &a, hipStream_t st) {
   if (a.in_mode != IN_BLK || a.W % a.blk_c != 0 || !col_pass_supported(a.H, a.blk_c)) return -3;
+  if (wave_pass_supported(a.H)) return launch_wave_col_pass(a, st);
   if (a.mid_tf) {
     if (a.out_mode != OUT_BLK) return -4;
     return dispatch<true, BLK_MID>(a, a.H, st);

@@ -144,4 +144,13 @@ template <int N, int ROWS, int RS, int ES, int NT> struct LdsFft {
   }
 };
 
+// Bijective workgroup -> tile map that gives each of the 8 XCDs (workgroups are dispatched
+// round-robin over them) a contiguous run of tiles, so neighbouring rows written into the
+// column-blocked layout share an L2.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  const int n8 = n & ~7;
+  if (b >= n8) return b;
+  return (b & 7) * (n8 >> 3) + (b >> 3);
+}
+
 }  // namespace hfx

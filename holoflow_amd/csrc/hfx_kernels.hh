@@ -37,6 +37,10 @@ struct DftLaunch {
   const uint16_t *wm_f16hi, *wm_f16lo, *wm_bf16;
 };
 int launch_time_dft(const DftLaunch &L, hipStream_t st);
+// Fused time DFT + forward row FFT into the column-blocked layout (L.planes is blocked).
+bool dft_row_supported(int W, int nbc, int T);
+int launch_time_dft_row(const DftLaunch &L, int H, int blk_c, int n_planes, const float2 *tw,
+                        hipStream_t st);
 
 // ---- 2-D FFT passes (fft2d.hip)
 enum InMode { IN_NAT_C = 0, IN_NAT_R = 1, IN_BLK = 2 };
@@ -78,6 +82,10 @@ int launch_col_pass(const PassArgs &a, hipStream_t st);
 bool row_pass_supported(int W);
 bool col_pass_supported(int H, int blk_c);
 int pick_blk_c(int W, int H);
+// wave-level FFT passes (wpass.hip) for lengths 256, 512, 768, 1024, 2048
+bool wave_pass_supported(int n);
+int launch_wave_row_pass(const PassArgs &a, hipStream_t st);
+int launch_wave_col_pass(const PassArgs &a, hipStream_t st);
 
 // ---- small ops (ops.hip)
 int launch_fftshift(const void *in, void *out, int elem_size, int ndim, const long long *shape,

@@ -1,0 +1,201 @@
+// Wave-level FFT engine: one 64-lane wavefront owns a whole transform of length
+// N = 64 * E (E in {4, 8, 12, 16, 32}), E elements per lane.
+//
+// Convention: on entry lane l holds x[l + 64*m] in v[m]; on return it holds X[l + 64*m].
+// Consecutive lanes therefore read and write consecutive indices (coalesced global
+// access), and the data never leaves the wave:
+//
+//   pass 1 : radix-E DFT entirely in registers (butterfly j = lane, Stockham NS = 1)
+//   pass k : radix-R Stockham pass through a wave-private LDS scratch (R <= 16), the
+//            last pass leaving its results in registers in the natural layout.
+//
+// For N = 1024 this is radix 16 (registers) -> radix 16 -> radix 4: two LDS round trips
+// and no workgroup barrier, against five barrier-separated radix-4 LDS passes in a
+// workgroup-wide Stockham.  Within a wave, LDS instructions complete in issue order, so
+// a write followed by a read of another lane's element needs only a compiler barrier.
+//
+// The scratch is padded by one element per 16 (pad(i) = i + i/16), which makes the
+// stride-E writes of pass 1 and the stride-NS accesses of the later passes free of bank
+// conflicts.  Pass twiddles come from an FP64-generated table tw[m] = exp(-2 pi i m/N);
+// the small in-register DFTs use the exact constants cos/sin(2 pi m/32) and (2 pi m/12).
+#pragma once
+
+#include "hfx_device.hh"
+
+namespace hfx {
+
+// cos(2*pi*m/32), m = 0..8
+__device__ constexpr float kCos32[9] = {1.0f,
+                                        0.98078528040323044913f,
+                                        0.92387953251128675613f,
+                                        0.83146961230254523708f,
+                                        0.70710678118654752440f,
+                                        0.55557023301960222474f,
+                                        0.38268343236508977173f,
+                                        0.19509032201612826785f,
+                                        0.0f};
+
+__device__ __forceinline__ constexpr float cos32(int m) {  // cos(2*pi*m/32), any m >= 0
+  m &= 31;
+  if (m <= 8) return kCos32[m];
+  if (m <= 16) return -kCos32[16 - m];
+  if (m <= 24) return -kCos32[m - 16];
+  return kCos32[32 - m];
+}
+__device__ __forceinline__ constexpr float sin32(int m) { return cos32(m + 24); }  // sin x = cos(x - pi/2)
+
+__device__ __forceinline__ constexpr float cos12(int m) {  // cos(2*pi*m/12)
+  constexpr float c[12] = {1.0f, 0.86602540378443864676f, 0.5f, 0.0f, -0.5f, -0.86602540378443864676f,
+                           -1.0f, -0.86602540378443864676f, -0.5f, 0.0f, 0.5f, 0.86602540378443864676f};
+  return c[m % 12];
+}
+__device__ __forceinline__ constexpr float sin12(int m) { return cos12(m + 9); }
+
+// exp(sgn * 2*pi*i * p / R) for a compile-time (after unrolling) exponent p
+template <int R> __device__ __forceinline__ float2 wr(int p, float sgn) {
+  p %= R;
+  if constexpr (32 % R == 0) {
+    const int m = p * (32 / R);
+    return make_float2(cos32(m), sgn * sin32(m));
+  } else {
+    static_assert(12 % R == 0, "unsupported in-register radix");
+    const int m = p * (12 / R);
+    return make_float2(cos12(m), sgn * sin12(m));
+  }
+}
+
+// In-register DFT of length R (natural order in and out), kernel exp(sgn*2*pi*i*k*n/R).
+// Composite lengths split as R = A*B (A = 4, 2 or 3):
+//   Y[n2][k1] = W_R^{n2 k1} * DFT_A(x[B n1 + n2])[k1],  X[k1 + A k2] = DFT_B(Y[.][k1])[k2].
+template <int R> struct RegDft {
+  __device__ __forceinline__ static void run(float2 (&v)[R], float sgn) {
+    if constexpr (R == 2 || R == 3 || R == 4) {
+      Butterfly<R>::run(v, sgn);
+    } else {
+      constexpr int A = (R % 4 == 0) ? 4 : (R % 2 == 0) ? 2 : 3;
+      constexpr int B = R / A;
+      float2 y[B][A];
+#pragma unroll
+      for (int n2 = 0; n2 < B; ++n2) {
+        float2 t[A];
+#pragma unroll
+        for (int n1 = 0; n1 < A; ++n1) t[n1] = v[B * n1 + n2];
+        Butterfly<A>::run(t, sgn);
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) y[n2][k1] = (n2 * k1 == 0) ? t[k1] : cmul(t[k1], wr<R>(n2 * k1, sgn));
+      }
+#pragma unroll
+      for (int k1 = 0; k1 < A; ++k1) {
+        float2 u[B];
+#pragma unroll
+        for (int n2 = 0; n2 < B; ++n2) u[n2] = y[n2][k1];
+        RegDft<B>::run(u, sgn);
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) v[k1 + A * k2] = u[k2];
+      }
+    }
+  }
+};
+
+__host__ __device__ constexpr int wave_pass_radix(int rem, int E) {
+  // largest power of two <= 16 dividing both the remaining length and E (so that every
+  // lane owns a whole number of butterflies)
+  int r = 16;
+  while (r > 1 && (rem % r != 0 || E % r != 0)) r /= 2;
+  return r;
+}
+
+template <int N> struct WaveFft {
+  static_assert(N % 64 == 0, "wave FFT length must be a multiple of 64");
+  static constexpr int E = N / 64;
+  static constexpr int SCR = N + N / 16;  // padded scratch elements per transform
+  // Pass twiddles are stored pass-major: the pass with stride NS and radix R owns
+  // (R-1)*NS entries W_{NS*R}^{r*k} at [r-1][k], so the lanes of a wave (consecutive k)
+  // read consecutive entries.  Over all passes this is N - E entries.
+  static constexpr int TW = N - E;
+
+  __device__ __forceinline__ static int pad(int i) { return i + (i >> 4); }
+
+  // offset of the table of the pass that starts at stride NS
+  static constexpr int tw_offset(int ns) {
+    int off = 0;
+    for (int s = E; s < ns;) {
+      const int r = wave_pass_radix(N / s, E);
+      off += (r - 1) * s;
+      s *= r;
+    }
+    return off;
+  }
+
+  // Builds the pass-major table from an N-entry table t[m] = exp(-2 pi i m / N).
+  __device__ static void build_twiddles(float2 *dst, const float2 *t, int tid, int nthreads) {
+    for (int s = E; s < N;) {
+      const int r = wave_pass_radix(N / s, E);
+      const int off = tw_offset(s);
+      for (int i = tid; i < (r - 1) * s; i += nthreads) {
+        const int rr = i / s + 1, k = i % s;
+        dst[off + i] = t[rr * k * (N / (s * r))];
+      }
+      s *= r;
+    }
+  }
+
+  template <int NS>
+  __device__ __forceinline__ static void pass(float2 (&v)[E], float2 *scr, const float2 *tw, int lane,
+                                              float sgn) {
+    constexpr int R = wave_pass_radix(N / NS, E);
+    static_assert(R >= 2, "bad factorisation");
+    constexpr int NB = N / R;
+    constexpr int PER = NB / 64;  // butterflies per lane
+    constexpr bool LAST = NS * R == N;
+    constexpr int TOFF = tw_offset(NS);
+    float2 u[PER][R];
+#pragma unroll
+    for (int q = 0; q < PER; ++q)
+#pragma unroll
+      for (int r = 0; r < R; ++r) u[q][r] = scr[pad(lane + 64 * q + r * NB)];
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int j = lane + 64 * q;
+      const int k = j % NS;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 w = tw[TOFF + (r - 1) * NS + k];
+        u[q][r] = cmul(u[q][r], make_float2(w.x, -sgn * w.y));
+      }
+      RegDft<R>::run(u[q], sgn);
+    }
+    if constexpr (LAST) {
+      // outputs j + r*NS with NS = N/R: lane holds X[lane + 64*(q + r*E/R)]
+#pragma unroll
+      for (int q = 0; q < PER; ++q)
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[q + r * (E / R)] = u[q][r];
+    } else {
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int j = lane + 64 * q;
+        const int base = (j / NS) * NS * R + j % NS;
+#pragma unroll
+        for (int r = 0; r < R; ++r) scr[pad(base + r * NS)] = u[q][r];
+      }
+      __builtin_amdgcn_wave_barrier();
+      pass<NS * R>(v, scr, tw, lane, sgn);
+    }
+  }
+
+  // Transforms v in place (see the convention above); scr: SCR wave-private LDS elements;
+  // tw: the TW-entry pass-major table from build_twiddles; inv: kernel exp(+...), unscaled.
+  __device__ __forceinline__ static void run(float2 (&v)[E], float2 *scr, const float2 *tw, int lane,
+                                             bool inv) {
+    const float sgn = inv ? 1.0f : -1.0f;
+    RegDft<E>::run(v, sgn);  // pass 1: butterfly j = lane over x[lane + 64 r]
+#pragma unroll
+    for (int r = 0; r < E; ++r) scr[pad(lane * E + r)] = v[r];
+    __builtin_amdgcn_wave_barrier();
+    pass<E>(v, scr, tw, lane, sgn);
+  }
+};
+
+}  // namespace hfx

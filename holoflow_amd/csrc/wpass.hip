@@ -1,0 +1,336 @@
+// Row and column passes built on the wave-level FFT (wfft.hh) for transform lengths
+// that are multiples of 64 (256, 512, 768, 1024, 2048).  Same kernel kinds and layouts
+// as the workgroup-wide passes in fft2d.hip, which remain for short lengths:
+//
+//   row pass    : one wave per image row; a workgroup holds 4 consecutive rows (tiles are
+//                 spread over the XCDs in contiguous runs).  BLK_TO_MOM waves loop over
+//                 all planes of their row, prefetching the next plane into registers.
+//   column pass : one wave per column of a column block (C <= 4 columns).  BLK_TO_MOM
+//                 workgroups hold S plane subsets x 4 columns (S = 2, or 1 for 2048 rows);
+//                 the subsets' partial moment sums are added in a fixed order at the
+//                 end, so results are deterministic.
+#include "hfx_kernels.hh"
+#include "wfft.hh"
+
+namespace hfx {
+
+namespace {
+
+// BLK_TO_MOM5: moments plus the two band-split M0 images (5 sums instead of 3)
+enum Kind { NAT_TO_BLK = 0, BLK_TO_MOM = 1, BLK_MID = 2, BLK_TO_NAT = 3, BLK_TO_MOM5 = 4 };
+constexpr bool is_mom(int k) { return k == BLK_TO_MOM || k == BLK_TO_MOM5; }
+
+__device__ __forceinline__ long long blk_index(int y, int x, int H, int C) {
+  return (long long)(x / C) * H * C + (long long)y * C + (x % C);
+}
+
+template <int E, int NACC> struct Moments {
+  float s[NACC][E];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int k = 0; k < NACC; ++k)
+#pragma unroll
+      for (int m = 0; m < E; ++m) s[k][m] = 0.f;
+  }
+  __device__ __forceinline__ void add(const float2 (&v)[E], const MomentArgs &mo, int plane) {
+    const float f = mo.f[plane], f2 = mo.f2[plane];
+    const bool lo = plane < mo.split;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      float a = v[m].x * v[m].x + v[m].y * v[m].y;
+      if (!mo.power2) a = sqrtf(a);
+      s[0][m] += a;
+      s[1][m] += __fmul_rn(f, a);
+      s[2][m] += __fmul_rn(f2, a);
+      if constexpr (NACC == 5) {
+        if (lo)
+          s[3][m] += a;
+        else
+          s[4][m] += a;
+      }
+    }
+  }
+};
+
+__device__ __forceinline__ void store_moment(const PassArgs &a, int nacc, int y, int x, const float (&val)[5]) {
+  int ys = y, xs = x;
+  if (a.mom.shift) {
+    ys = (y + a.H - a.H / 2) % a.H;
+    xs = (x + a.W - a.W / 2) % a.W;
+  }
+  const long long o = (long long)ys * a.W + xs;
+  const float n_all = (float)a.planes;
+  a.mom.out[0][o] = val[0] / n_all;
+  a.mom.out[1][o] = val[1] / n_all;
+  a.mom.out[2][o] = val[2] / n_all;
+  if (nacc == 5) {
+    if (a.mom.out[3]) a.mom.out[3][o] = val[3] / (float)a.mom.split;
+    if (a.mom.out[4]) a.mom.out[4][o] = val[4] / (float)(a.planes - a.mom.split);
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------- row pass
+constexpr int WROW_WAVES = 4;
+
+template <int N, int KIND>
+__global__ __launch_bounds__(64 * WROW_WAVES) void k_wrow(PassArgs a) {
+  using F = WaveFft<N>;
+  constexpr int E = F::E;
+  __shared__ float2 scr[WROW_WAVES * F::SCR];
+  __shared__ float2 twl[F::TW];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  F::build_twiddles(twl, a.tw, tid, 64 * WROW_WAVES);
+  __syncthreads();
+
+  const int y = xcd_tile(blockIdx.x, gridDim.x) * WROW_WAVES + wave;
+  if (y >= a.H) return;
+  const int C = a.blk_c;
+  const long long plane_elems = (long long)a.H * a.W;
+  float2 *ws = scr + wave * F::SCR;
+
+  auto load = [&](float2 (&v)[E], long long p) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int x = lane + 64 * m;
+      if (KIND == NAT_TO_BLK) {
+        const int yi = y - a.off_y, xi = x - a.off_x;
+        float2 t = make_float2(0.f, 0.f);
+        if (yi >= 0 && yi < a.in_h && xi >= 0 && xi < a.in_w) {
+          const long long o = p * (long long)a.in_h * a.in_w + (long long)yi * a.in_w + xi;
+          if (a.in_mode == IN_NAT_C)
+            t = reinterpret_cast<const float2 *>(a.in)[o];
+          else
+            t = make_float2(reinterpret_cast<const float *>(a.in)[o], 0.f);
+          if (a.pre_lx) t = cmul(t, cmul(a.pre_lx[xi], a.pre_ly[yi]));
+        }
+        v[m] = t;
+      } else {
+        v[m] = reinterpret_cast<const float2 *>(a.in)[p * plane_elems + blk_index(y, x, a.H, C)];
+      }
+    }
+  };
+
+  if constexpr (is_mom(KIND)) {
+    constexpr int NACC = KIND == BLK_TO_MOM5 ? 5 : 3;
+    Moments<E, NACC> acc;
+    acc.zero();
+    if constexpr (E <= 16) {
+      float2 v[E], nxt[E];
+      load(nxt, 0);
+      for (int p = 0; p < a.planes; ++p) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = nxt[m];
+        if (p + 1 < a.planes) load(nxt, p + 1);
+        F::run(v, ws, twl, lane, a.inv != 0);
+        acc.add(v, a.mom, p);
+      }
+    } else {  // 2048-point rows: no register room for a prefetch next to 5*E sums
+      for (int p = 0; p < a.planes; ++p) {
+        float2 v[E];
+        load(v, p);
+        F::run(v, ws, twl, lane, a.inv != 0);
+        acc.add(v, a.mom, p);
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      float val[5] = {acc.s[0][m], acc.s[1][m], acc.s[2][m], 0.f, 0.f};
+      if constexpr (NACC == 5) {
+        val[3] = acc.s[3][m];
+        val[4] = acc.s[4][m];
+      }
+      store_moment(a, NACC, y, lane + 64 * m, val);
+    }
+  } else {
+    const long long p = blockIdx.y;
+    float2 v[E];
+    load(v, p);
+    F::run(v, ws, twl, lane, a.inv != 0);
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int x = lane + 64 * m;
+      float2 t = v[m];
+      if (KIND == NAT_TO_BLK) {
+        reinterpret_cast<float2 *>(a.out)[p * plane_elems + blk_index(y, x, a.H, C)] = t;
+      } else {
+        const long long o = p * plane_elems + (long long)y * a.W + x;
+        if (a.out_mode == OUT_NAT_MAG) {
+          reinterpret_cast<float *>(a.out)[o] = sqrtf(t.x * t.x + t.y * t.y);
+        } else {
+          if (a.post_lx) t = cmul(t, cmul(a.post_lx[x], a.post_ly[y]));
+          reinterpret_cast<float2 *>(a.out)[o] = t;
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------- column pass
+// Plane subsets per column tile (waves = 4 * subsets).  A wave keeps its column (E
+// values), the next plane's prefetch and 5*E moment sums in registers, which allows 2
+// waves per SIMD up to 1024 rows and 1 for 2048.
+constexpr int wcol_subsets(int n) { return n >= 2048 ? 1 : 2; }
+
+template <int N, int KIND>
+__global__ __launch_bounds__(is_mom(KIND) ? 256 * wcol_subsets(N) : 256) void k_wcol(PassArgs a) {
+  using F = WaveFft<N>;
+  constexpr int E = F::E;
+  constexpr int S = is_mom(KIND) ? wcol_subsets(N) : 1;
+  constexpr int WAVES = 4 * S;
+  __shared__ float2 scr[WAVES * F::SCR];
+  __shared__ float2 twl[F::TW];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  F::build_twiddles(twl, a.tw, tid, 64 * WAVES);
+  __syncthreads();
+
+  const int C = a.blk_c;
+  const int c = wave & 3, sub = wave >> 2;
+  const int x = blockIdx.x * C + c;
+  const bool active = c < C;
+  const long long plane_elems = (long long)a.H * a.W;
+  const long long tile = (long long)blockIdx.x * a.H * C;  // blocked offset of the tile
+  float2 *ws = scr + wave * F::SCR;
+
+  auto load = [&](float2 (&v)[E], long long p) {
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+      v[m] = reinterpret_cast<const float2 *>(a.in)[p * plane_elems + tile + (long long)(lane + 64 * m) * C + c];
+  };
+
+  if constexpr (is_mom(KIND)) {
+    constexpr int NACC = KIND == BLK_TO_MOM5 ? 5 : 3;
+    Moments<E, NACC> acc;
+    acc.zero();
+    if (active) {
+      float2 v[E], nxt[E];
+      if (sub < a.planes) load(nxt, sub);
+      for (int p = sub; p < a.planes; p += S) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[m] = nxt[m];
+        if (p + S < a.planes) load(nxt, p + S);
+        F::run(v, ws, twl, lane, a.inv != 0);
+        acc.add(v, a.mom, p);
+      }
+    }
+    // fixed-order combination of the plane subsets: sub 0 += sub 1, sub 2, ... (LDS reused)
+    float *red = reinterpret_cast<float *>(scr);  // [(S-1)][4 columns][N]
+    __syncthreads();
+    // k is a compile-time index after unrolling: a runtime index into acc.s would move
+    // the sums to scratch memory
+#pragma unroll
+    for (int k = 0; k < NACC; ++k) {
+      if (S == 1) continue;
+      if (sub > 0 && active) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) red[((sub - 1) * 4 + c) * N + lane + 64 * m] = acc.s[k][m];
+      }
+      __syncthreads();
+      if (sub == 0 && active) {
+        for (int s = 1; s < S; ++s)
+#pragma unroll
+          for (int m = 0; m < E; ++m) acc.s[k][m] += red[((s - 1) * 4 + c) * N + lane + 64 * m];
+      }
+      __syncthreads();
+    }
+    if (sub == 0 && active) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        float val[5] = {acc.s[0][m], acc.s[1][m], acc.s[2][m], 0.f, 0.f};
+        if constexpr (NACC == 5) {
+          val[3] = acc.s[3][m];
+          val[4] = acc.s[4][m];
+        }
+        store_moment(a, NACC, lane + 64 * m, x, val);
+      }
+    }
+  } else {
+    if (!active) return;
+    const long long p = blockIdx.y;
+    float2 v[E];
+    load(v, p);
+    if (KIND == BLK_MID) {
+      F::run(v, ws, twl, lane, false);
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.mid_tf[tile + (long long)(lane + 64 * m) * C + c]);
+      F::run(v, ws, twl, lane, true);
+#pragma unroll
+      for (int m = 0; m < E; ++m)
+        reinterpret_cast<float2 *>(a.out)[p * plane_elems + tile + (long long)(lane + 64 * m) * C + c] = v[m];
+    } else {
+      F::run(v, ws, twl, lane, a.inv != 0);
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int y = lane + 64 * m;
+        float2 t = v[m];
+        const long long o = p * plane_elems + (long long)y * a.W + x;
+        if (a.out_mode == OUT_NAT_MAG) {
+          reinterpret_cast<float *>(a.out)[o] = sqrtf(t.x * t.x + t.y * t.y);
+        } else {
+          if (a.post_lx) t = cmul(t, cmul(a.post_lx[x], a.post_ly[y]));
+          reinterpret_cast<float2 *>(a.out)[o] = t;
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------- launch
+#define HFX_WAVE_SIZES(X) X(256) X(512) X(768) X(1024) X(2048)
+
+bool wave_pass_supported(int n) { return n == 256 || n == 512 || n == 768 || n == 1024 || n == 2048; }
+
+template <int N, int KIND> static int launch_wrow_n(const PassArgs &a, hipStream_t st) {
+  const unsigned gx = (unsigned)((a.H + WROW_WAVES - 1) / WROW_WAVES);
+  const unsigned gy = is_mom(KIND) ? 1u : (unsigned)a.planes;
+  hipLaunchKernelGGL((k_wrow<N, KIND>), dim3(gx, gy), dim3(64 * WROW_WAVES), 0, st, a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int N, int KIND> static int launch_wcol_n(const PassArgs &a, hipStream_t st) {
+  const unsigned gx = (unsigned)(a.W / a.blk_c);
+  const unsigned gy = is_mom(KIND) ? 1u : (unsigned)a.planes;
+  const unsigned nt = is_mom(KIND) ? 256 * wcol_subsets(N) : 256;
+  hipLaunchKernelGGL((k_wcol<N, KIND>), dim3(gx, gy), dim3(nt), 0, st, a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int KIND> static int wrow_dispatch(const PassArgs &a, hipStream_t st) {
+#define X(s) if (a.W == s) return launch_wrow_n<s, KIND>(a, st);
+  HFX_WAVE_SIZES(X)
+#undef X
+  return -2;
+}
+
+template <int KIND> static int wcol_dispatch(const PassArgs &a, hipStream_t st) {
+#define X(s) if (a.H == s) return launch_wcol_n<s, KIND>(a, st);
+  HFX_WAVE_SIZES(X)
+#undef X
+  return -2;
+}
+
+int launch_wave_row_pass(const PassArgs &a, hipStream_t st) {
+  if (a.in_mode != IN_BLK) {
+    if (a.out_mode != OUT_BLK) return -4;
+    return wrow_dispatch<NAT_TO_BLK>(a, st);
+  }
+  if (a.out_mode == OUT_MOMENTS)
+    return a.mom.split > 0 ? wrow_dispatch<BLK_TO_MOM5>(a, st) : wrow_dispatch<BLK_TO_MOM>(a, st);
+  if (a.out_mode == OUT_NAT_C || a.out_mode == OUT_NAT_MAG) return wrow_dispatch<BLK_TO_NAT>(a, st);
+  return -4;
+}
+
+int launch_wave_col_pass(const PassArgs &a, hipStream_t st) {
+  if (a.in_mode != IN_BLK || a.blk_c < 1 || a.blk_c > 4 || a.W % a.blk_c != 0) return -3;
+  if (a.mid_tf) {
+    if (a.out_mode != OUT_BLK) return -4;
+    return wcol_dispatch<BLK_MID>(a, st);
+  }
+  if (a.out_mode == OUT_MOMENTS)
+    return a.mom.split > 0 ? wcol_dispatch<BLK_TO_MOM5>(a, st) : wcol_dispatch<BLK_TO_MOM>(a, st);
+  if (a.out_mode == OUT_NAT_C || a.out_mode == OUT_NAT_MAG) return wcol_dispatch<BLK_TO_NAT>(a, st);
+  return -4;
+}
+
+}  // namespace hfx

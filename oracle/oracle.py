@@ -265,8 +265,12 @@ def doppler_moments(window: np.ndarray, p: PathParams, prec: str = "f64") -> np.
 
 
 def rel_l2(x, ref) -> float:
-    x = np.asarray(x, dtype=np.float64)
-    ref = np.asarray(ref, dtype=np.float64)
+    """||x - ref|| / ||ref|| in float64 (complex128 when either side is complex)."""
+    x = np.asarray(x)
+    ref = np.asarray(ref)
+    dt = np.complex128 if (np.iscomplexobj(x) or np.iscomplexobj(ref)) else np.float64
+    x = x.astype(dt)
+    ref = ref.astype(dt)
     den = np.linalg.norm(ref)
     return float(np.linalg.norm(x - ref) / (den if den > 0 else 1.0))
 

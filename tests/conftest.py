@@ -30,7 +30,10 @@ def gpu():
 def rel_l2(x, ref):
     import numpy as np
 
-    x = np.asarray(x, dtype=np.float64)
-    ref = np.asarray(ref, dtype=np.float64)
+    x = np.asarray(x)
+    ref = np.asarray(ref)
+    dt = np.complex128 if (np.iscomplexobj(x) or np.iscomplexobj(ref)) else np.float64
+    x = x.astype(dt)
+    ref = ref.astype(dt)
     d = np.linalg.norm(ref)
     return float(np.linalg.norm(x - ref) / (d if d > 0 else 1.0))
