@@ -18,6 +18,8 @@ HFX_E_INVALID = 1
 HFX_E_UNSUPPORTED = 2
 HFX_E_HIP = 3
 HFX_E_NOMEM = 4
+HFX_NOT_READY = 5
+HFX_EOF = 6
 
 PROP_FRESNEL = 0
 PROP_ANGULAR_SPECTRUM = 1
@@ -35,6 +37,9 @@ FLAG_BOTH_SIDEBANDS = 4
 DTYPE_U16 = 1
 DTYPE_F32 = 2
 DTYPE_CF32 = 3
+
+REDUCE_POWER2 = 1
+REDUCE_ABS = 2
 
 
 class HfxError(RuntimeError):
@@ -98,7 +103,7 @@ class SynthDesc(C.Structure):
 
 _VP = C.c_void_p
 _P = C.POINTER
-# name -> (restype, argtypes); every symbol declared in include/hfx.h
+# name -> (restype, argtypes); every symbol declared in include/hfx.h and include/hfx_nodes.h
 SIGNATURES = {
     "hfx_last_error": (C.c_char_p, []),
     "hfx_abi_version": (C.c_int32, []),
@@ -110,7 +115,8 @@ SIGNATURES = {
     "hfx_moments": (C.c_int, [_VP, _VP, C.c_int64, C.c_int64, _VP, _VP]),
     "hfx_moments_batch": (C.c_int, [_VP, _VP, C.c_int64, C.c_int64, C.c_int32, C.c_int64, _VP, _VP]),
     "hfx_plan_time_dft": (C.c_int, [_VP, _VP, C.c_int64, C.c_int64, _VP, _VP]),
-    "hfx_dft_axis": (C.c_int, [_VP, C.c_int32, C.c_int64, C.c_int32, C.c_int64, C.c_int32, C.c_int32, _VP, _VP]),
+    "hfx_dft_axis": (C.c_int, [_VP, C.c_int32, C.c_int64, C.c_int32, C.c_int64, C.c_int32, C.c_int32,
+                               C.c_float, _VP, _VP]),
     "hfx_fresnel_create": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,
                                      C.c_double, C.c_int32, C.c_int32, C.c_int32, _VP, _P(_VP)]),
     "hfx_fresnel_update_z": (C.c_int, [_VP, C.c_double, _VP]),
@@ -119,11 +125,26 @@ SIGNATURES = {
     "hfx_asm_create": (C.c_int, [_P(AsmDesc), _VP, _P(_VP)]),
     "hfx_asm_execute": (C.c_int, [_VP, _VP, _VP, _VP]),
     "hfx_asm_destroy": (C.c_int, [_VP]),
-    "hfx_weighted_mean": (C.c_int, [_VP, C.c_int32, C.c_int64, C.c_int64, _VP, C.c_int32, _VP, _VP]),
+    "hfx_weighted_mean": (C.c_int, [_VP, C.c_int32, C.c_int64, C.c_int64, _VP, C.c_uint32, _VP, _VP]),
+    "hfx_strided_copy": (C.c_int, [_VP, _P(C.c_int64), _P(C.c_int64), C.c_int64, C.c_int32, C.c_int32, _VP,
+                                   C.c_int64, C.c_int64, _VP]),
+    "hfx_multiply": (C.c_int, [_VP, C.c_int32, _P(C.c_int64), _VP, C.c_int32, _P(C.c_int64), C.c_int32,
+                               _P(C.c_int64), _VP, _VP]),
     "hfx_fftshift": (C.c_int, [_VP, _VP, C.c_int32, C.c_int32, _P(C.c_int64), _P(C.c_int32), _VP]),
     "hfx_convert_u16": (C.c_int, [_VP, _VP, C.c_int64, C.c_int32, _VP]),
     "hfx_synth_desc_init": (None, [_P(SynthDesc), C.c_int32, C.c_int32]),
     "hfx_synth_frames": (C.c_int, [_P(SynthDesc), _VP, C.c_int64, C.c_int64, _VP, _VP]),
+    # include/hfx_nodes.h
+    "hfx_node_kinds": (C.c_int, [C.c_char_p, C.c_int64]),
+    "hfx_node_infer": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int64]),
+    "hfx_node_create": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, _VP, _P(_VP)]),
+    "hfx_node_update": (C.c_int, [_VP, C.c_char_p, C.c_char_p, _VP, _P(C.c_int32)]),
+    "hfx_node_execute": (C.c_int, [_VP, _P(_VP), C.c_int32, _P(_VP), C.c_int32]),
+    "hfx_node_acquire_input": (C.c_int, [_VP, _P(_VP)]),
+    "hfx_node_push": (C.c_int, [_VP]),
+    "hfx_node_pop": (C.c_int, [_VP, _P(_VP)]),
+    "hfx_node_release_output": (C.c_int, [_VP]),
+    "hfx_node_destroy": (C.c_int, [_VP]),
 }
 
 _lib = None

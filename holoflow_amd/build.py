@@ -17,12 +17,12 @@ ROOT = os.path.dirname(HERE)
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhfx.so")
 
-SOURCES = ["tables.hip", "dft.hip", "fft2d.hip", "wpass.hip", "ops.hip", "synth.hip", "api.hip"]
+SOURCES = ["tables.hip", "dft.hip", "fft2d.hip", "wpass.hip", "ops.hip", "synth.hip", "api.hip", "nodes.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
-    "-std=c++17",
+    "-std=c++20",
     "-fPIC",
     "-Wall",
     "-Wno-unused-function",

@@ -19,6 +19,14 @@ namespace {
 
 thread_local std::string g_err;
 
+}  // namespace
+
+namespace hfx {
+void set_last_error(const std::string &msg) { g_err = msg; }
+}  // namespace hfx
+
+namespace {
+
 hfx_status fail(hfx_status code, const std::string &msg) {
   g_err = msg;
   return code;
@@ -401,12 +409,12 @@ hfx_status hfx_moments_batch(hfx_plan *p, const uint16_t *ring, int64_t ring_fra
 // ------------------------------------------------------------------------ per-node kinds
 
 hfx_status hfx_dft_axis(const void *in, int32_t in_dtype, int64_t outer, int32_t n, int64_t inner,
-                        int32_t k_begin, int32_t k_end, void *out, hfx_stream_t stream) {
+                        int32_t k_begin, int32_t k_end, float scale, void *out, hfx_stream_t stream) {
   if (!in || !out) return fail(HFX_E_INVALID, "null argument");
   if (in_dtype < 1 || in_dtype > 3) return fail(HFX_E_INVALID, "unsupported input dtype");
   if (n <= 0 || outer < 0 || inner < 0) return fail(HFX_E_INVALID, "invalid shape");
   if (k_begin < 0 || k_end < k_begin || k_end > n) return fail(HFX_E_INVALID, "invalid bin range");
-  if (hfx::launch_dft_axis(in, in_dtype, outer, n, inner, k_begin, k_end,
+  if (hfx::launch_dft_axis(in, in_dtype, outer, n, inner, k_begin, k_end, scale,
                            reinterpret_cast<float2 *>(out), S(stream)) != 0)
     return fail(HFX_E_HIP, "dft launch failed");
   return HFX_OK;
@@ -574,24 +582,62 @@ hfx_status hfx_asm_destroy(hfx_asm *a) {
 }
 
 hfx_status hfx_weighted_mean(const void *in, int32_t in_complex, int64_t planes, int64_t npix,
-                             const float *weights, int32_t power2, float *out,
-                             hfx_stream_t stream) {
+                             const float *weights, uint32_t flags, float *out, hfx_stream_t stream) {
   if (!in || !out) return fail(HFX_E_INVALID, "null argument");
   if (planes <= 0 || npix <= 0) return fail(HFX_E_INVALID, "reduction has zero elements");
+  const int power2 = (flags & HFX_REDUCE_POWER2) ? 1 : 0;
+  const int absval = (flags & HFX_REDUCE_ABS) ? 1 : 0;
   float *outs[3] = {out, nullptr, nullptr};
   int rc;
   if (weights) {
-    // out = mean_k(w_k * m_k): route the weights through the M1 slot of the reducer and
-    // discard M0 into the same buffer first.
-    outs[0] = out;
+    // out = mean_k(w_k * m_k): the weighted sum is the reducer's second output; its first
+    // (unweighted) output is written to the same buffer first and then overwritten.
     outs[1] = out;
     rc = hfx::launch_moments_reduce(in, in_complex, planes, npix, weights, weights, outs, 2, power2,
-                                    S(stream));
+                                    absval, S(stream));
   } else {
     rc = hfx::launch_moments_reduce(in, in_complex, planes, npix, nullptr, nullptr, outs, 1, power2,
-                                    S(stream));
+                                    absval, S(stream));
   }
   if (rc != 0) return fail(HFX_E_HIP, "reduce launch failed");
+  return HFX_OK;
+}
+
+hfx_status hfx_strided_copy(const void *in, const int64_t *shape, const int64_t *in_strides_bytes,
+                            int64_t in_offset_bytes, int32_t ndim, int32_t elem_size, void *out,
+                            int64_t out_block, int64_t out_pitch, hfx_stream_t stream) {
+  if (!in || !out || !shape || !in_strides_bytes) return fail(HFX_E_INVALID, "null argument");
+  if (ndim < 1 || ndim > 8) return fail(HFX_E_INVALID, "rank must be in [1, 8]");
+  if (elem_size <= 0 || in_offset_bytes < 0) return fail(HFX_E_INVALID, "invalid element size or offset");
+  if (out_block < 0 || (out_block > 0 && out_pitch < out_block))
+    return fail(HFX_E_INVALID, "out_pitch must be at least out_block");
+  long long shp[8], st[8];
+  for (int i = 0; i < ndim; ++i) {
+    if (shape[i] < 0 || in_strides_bytes[i] < 0) return fail(HFX_E_INVALID, "negative shape or stride");
+    shp[i] = shape[i];
+    st[i] = in_strides_bytes[i];
+  }
+  if (hfx::launch_strided_copy(in, out, elem_size, ndim, shp, st, in_offset_bytes, out_block, out_pitch,
+                               S(stream)) != 0)
+    return fail(HFX_E_HIP, "copy launch failed");
+  return HFX_OK;
+}
+
+hfx_status hfx_multiply(const void *a, int32_t a_complex, const int64_t *a_strides, const void *b,
+                        int32_t b_complex, const int64_t *b_strides, int32_t ndim,
+                        const int64_t *out_shape, void *out, hfx_stream_t stream) {
+  if (!a || !b || !out || !a_strides || !b_strides || !out_shape) return fail(HFX_E_INVALID, "null argument");
+  if (ndim < 1 || ndim > 8) return fail(HFX_E_INVALID, "rank must be in [1, 8]");
+  long long shp[8], sa[8], sb[8];
+  for (int i = 0; i < ndim; ++i) {
+    if (out_shape[i] < 0 || a_strides[i] < 0 || b_strides[i] < 0)
+      return fail(HFX_E_INVALID, "negative shape or stride");
+    shp[i] = out_shape[i];
+    sa[i] = a_strides[i];
+    sb[i] = b_strides[i];
+  }
+  if (hfx::launch_multiply(a, a_complex, sa, b, b_complex, sb, ndim, shp, out, S(stream)) != 0)
+    return fail(HFX_E_HIP, "multiply launch failed");
   return HFX_OK;
 }
 

@@ -92,10 +92,15 @@ int launch_fftshift(const void *in, void *out, int elem_size, int ndim, const lo
                     const int *shift_axes_mask, hipStream_t st);
 int launch_moments_reduce(const void *in, int in_complex, long long planes, long long npix,
                           const float *f, const float *f2, float **out, int n_out, int power2,
-                          hipStream_t st);
+                          int absval, hipStream_t st);
+int launch_strided_copy(const void *in, void *out, int elem_size, int ndim, const long long *shape,
+                        const long long *in_strides, long long in_offset, long long out_block,
+                        long long out_pitch, hipStream_t st);
+int launch_multiply(const void *a, int a_complex, const long long *sa, const void *b, int b_complex,
+                    const long long *sb, int ndim, const long long *shape, void *out, hipStream_t st);
 int launch_convert_u16(const uint16_t *in, void *out, long long n, int to_complex, hipStream_t st);
 int launch_dft_axis(const void *in, int dtype, long long outer, int n, long long inner, int kb,
-                    int ke, float2 *out, hipStream_t st);
+                    int ke, float scale, float2 *out, hipStream_t st);
 
 // ---- synthetic interferograms (synth.hip)
 struct SynthArgs {

@@ -63,7 +63,7 @@ struct ReduceArgs {
   long long planes, npix;
   const float *f, *f2;
   float *out[3];
-  int n_out, power2;
+  int n_out, power2, absval;
 };
 
 __global__ void k_moments_reduce(ReduceArgs a) {
@@ -78,7 +78,7 @@ __global__ void k_moments_reduce(ReduceArgs a) {
       if (!a.power2) m = sqrtf(m);
     } else {
       m = reinterpret_cast<const float *>(a.in)[k * a.npix + p];
-      m = a.power2 ? m * m : fabsf(m);
+      m = a.power2 ? m * m : (a.absval ? fabsf(m) : m);
     }
     s0 += m;
     if (a.n_out > 1) s1 += __fmul_rn(a.f[k], m);
@@ -92,8 +92,9 @@ __global__ void k_moments_reduce(ReduceArgs a) {
 
 int launch_moments_reduce(const void *in, int in_complex, long long planes, long long npix,
                           const float *f, const float *f2, float **out, int n_out, int power2,
-                          hipStream_t st) {
+                          int absval, hipStream_t st) {
   ReduceArgs a;
+  a.absval = absval;
   a.in = in;
   a.in_complex = in_complex;
   a.planes = planes;
@@ -124,6 +125,110 @@ int launch_convert_u16(const uint16_t *in, void *out, long long n, int to_comple
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// Gather of a strided view (holonp::Copy, copy.cu:40,117-125; the pieces of
+// holonp::Concatenate).  Strides and offset in bytes, as TDesc has them.  Element t of the
+// view goes to out[(t / out_block) * out_pitch + t % out_block]: one input of a
+// concatenation along an inner axis lands in its column band of the output in one launch.
+struct CopyGeo {
+  int ndim, esz;
+  long long shape[8];
+  long long in_stride[8];
+  long long in_offset;
+  long long total;
+  long long out_block, out_pitch;
+};
+
+__global__ void k_strided_copy(const uint8_t *in, uint8_t *out, CopyGeo g) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.total) return;
+  long long rem = t, src = g.in_offset;
+  for (int k = g.ndim - 1; k >= 0; --k) {
+    const long long idx = rem % g.shape[k];
+    rem /= g.shape[k];
+    src += idx * g.in_stride[k];
+  }
+  const long long o = (t / g.out_block) * g.out_pitch + t % g.out_block;
+  if (g.esz == 8)
+    reinterpret_cast<uint2 *>(out)[o] = *reinterpret_cast<const uint2 *>(in + src);
+  else if (g.esz == 4)
+    reinterpret_cast<uint32_t *>(out)[o] = *reinterpret_cast<const uint32_t *>(in + src);
+  else
+    for (int b = 0; b < g.esz; ++b) out[o * g.esz + b] = in[src + b];
+}
+
+int launch_strided_copy(const void *in, void *out, int elem_size, int ndim, const long long *shape,
+                        const long long *in_strides, long long in_offset, long long out_block,
+                        long long out_pitch, hipStream_t st) {
+  if (ndim < 1 || ndim > 8) return -1;
+  CopyGeo g;
+  g.ndim = ndim;
+  g.esz = elem_size;
+  g.in_offset = in_offset;
+  g.total = 1;
+  for (int k = 0; k < ndim; ++k) {
+    g.shape[k] = shape[k];
+    g.in_stride[k] = in_strides[k];
+    g.total *= shape[k];
+  }
+  if (g.total == 0) return 0;
+  g.out_block = out_block > 0 ? out_block : g.total;
+  g.out_pitch = out_block > 0 ? out_pitch : g.total;
+  hipLaunchKernelGGL(k_strided_copy, dim3((unsigned)((g.total + 255) / 256)), dim3(256), 0, st,
+                     reinterpret_cast<const uint8_t *>(in), reinterpret_cast<uint8_t *>(out), g);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Broadcasting element-wise product (holonp::Multiply, multiply.cu:69-92): out has the
+// broadcast shape; an operand's element stride is 0 on its broadcast axes.  Real operands
+// are promoted to x + 0i when the output is complex.
+struct MulGeo {
+  int ndim;
+  long long shape[8];
+  long long sa[8], sb[8];  // element strides
+  long long total;
+  int a_complex, b_complex, out_complex;
+};
+
+__global__ void k_multiply(const void *a, const void *b, void *out, MulGeo g) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.total) return;
+  long long rem = t, ia = 0, ib = 0;
+  for (int k = g.ndim - 1; k >= 0; --k) {
+    const long long idx = rem % g.shape[k];
+    rem /= g.shape[k];
+    ia += idx * g.sa[k];
+    ib += idx * g.sb[k];
+  }
+  const float2 va = g.a_complex ? reinterpret_cast<const float2 *>(a)[ia]
+                                : make_float2(reinterpret_cast<const float *>(a)[ia], 0.f);
+  const float2 vb = g.b_complex ? reinterpret_cast<const float2 *>(b)[ib]
+                                : make_float2(reinterpret_cast<const float *>(b)[ib], 0.f);
+  if (g.out_complex)
+    reinterpret_cast<float2 *>(out)[t] = cmul(va, vb);
+  else
+    reinterpret_cast<float *>(out)[t] = va.x * vb.x;
+}
+
+int launch_multiply(const void *a, int a_complex, const long long *sa, const void *b, int b_complex,
+                    const long long *sb, int ndim, const long long *shape, void *out, hipStream_t st) {
+  if (ndim < 1 || ndim > 8) return -1;
+  MulGeo g;
+  g.ndim = ndim;
+  g.total = 1;
+  for (int k = 0; k < ndim; ++k) {
+    g.shape[k] = shape[k];
+    g.sa[k] = sa[k];
+    g.sb[k] = sb[k];
+    g.total *= shape[k];
+  }
+  g.a_complex = a_complex;
+  g.b_complex = b_complex;
+  g.out_complex = a_complex || b_complex;
+  if (g.total == 0) return 0;
+  hipLaunchKernelGGL(k_multiply, dim3((unsigned)((g.total + 255) / 256)), dim3(256), 0, st, a, b, out, g);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 // Direct DFT along the middle axis of [outer][n][inner] for the stand-alone RFFT / FFT
 // kinds (any axis, any length).  Twiddles are evaluated in FP64 from the exactly
 // reduced integer argument (k*j mod n) and rounded to FP32; products and sums are FP32.
@@ -132,6 +237,7 @@ struct DftAxisArgs {
   int dtype;  // 1 U16, 2 F32, 3 CF32
   long long outer, inner;
   int n, kb, ke;
+  float scale;
   float2 *out;
 };
 
@@ -163,12 +269,12 @@ __global__ void k_dft_axis(DftAxisArgs a) {
     sr += xr * wr - xi * wi;
     si += xr * wi + xi * wr;
   }
-  a.out[t] = make_float2(sr, si);
+  a.out[t] = make_float2(sr * a.scale, si * a.scale);
 }
 
 int launch_dft_axis(const void *in, int dtype, long long outer, int n, long long inner, int kb,
-                    int ke, float2 *out, hipStream_t st) {
-  DftAxisArgs a{in, dtype, outer, inner, n, kb, ke, out};
+                    int ke, float scale, float2 *out, hipStream_t st) {
+  DftAxisArgs a{in, dtype, outer, inner, n, kb, ke, scale, out};
   long long total = outer * (long long)(ke - kb) * inner;
   if (total == 0) return 0;
   hipLaunchKernelGGL(k_dft_axis, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, a);

@@ -131,8 +131,8 @@ class MomentPlan:
 
 # ----------------------------------------------------------------------- per-node kinds
 def dft_axis(x: torch.Tensor, axis: int, k_begin: int = 0, k_end: Optional[int] = None,
-             stream=None) -> torch.Tensor:
-    """RFFT/FFT along `axis` (unscaled), bins [k_begin, k_end) -> complex64."""
+             scale: float = 1.0, stream=None) -> torch.Tensor:
+    """RFFT/FFT along `axis` (times `scale`), bins [k_begin, k_end) -> complex64."""
     axis = axis % x.dim()
     n = x.shape[axis]
     k_end = n if k_end is None else k_end
@@ -148,7 +148,7 @@ def dft_axis(x: torch.Tensor, axis: int, k_begin: int = 0, k_end: Optional[int] 
     shape = list(x.shape)
     shape[axis] = k_end - k_begin
     out = torch.empty(shape, dtype=torch.complex64, device="cuda")
-    check(lib().hfx_dft_axis(_ptr(x), dt, outer, n, inner, k_begin, k_end, _ptr(out),
+    check(lib().hfx_dft_axis(_ptr(x), dt, outer, n, inner, k_begin, k_end, scale, _ptr(out),
                              _stream(stream)))
     return out
 
@@ -211,14 +211,16 @@ class AngularSpectrum:
 
 
 def weighted_mean(x: torch.Tensor, weights: Optional[torch.Tensor] = None, power2=False,
-                  stream=None) -> torch.Tensor:
-    """Mean over axis 0 of |x| (complex) or x (real), optionally weighted per plane."""
+                  absval=False, stream=None) -> torch.Tensor:
+    """Mean over axis 0 of |x| (complex) or x (real; |x| with absval), optionally weighted
+    per plane."""
     planes = x.shape[0]
     npix = x[0].numel()
     out = torch.empty(x.shape[1:], dtype=torch.float32, device="cuda")
     w = C.c_void_p(weights.data_ptr()) if weights is not None else None
+    flags = (_lib.REDUCE_POWER2 if power2 else 0) | (_lib.REDUCE_ABS if absval else 0)
     check(lib().hfx_weighted_mean(_ptr(x.contiguous()), int(x.is_complex()), planes, npix, w,
-                                  int(power2), _ptr(out), _stream(stream)))
+                                  flags, _ptr(out), _stream(stream)))
     return out
 
 

@@ -117,12 +117,12 @@ hfx_status hfx_plan_time_dft(hfx_plan *p, const uint16_t *ring, int64_t ring_fra
  * Per-node kinds (drop-in replacements of individual reference tasks).
  * ---------------------------------------------------------------------------------- */
 
-/* holonp::RFFT / holonp::FFT along one axis (rfft.cu:148-297, fft.cu:234-386), norm
- * "backward" (unscaled), for a contiguous tensor viewed as [outer][n][inner] with the
- * transform over n.  in_dtype: 1 = U16, 2 = F32, 3 = CF32 (FFT only).  Outputs CF32
- * [outer][n_out][inner] holding bins [k_begin, k_end) (RFFT: k_end <= n/2+1). */
+/* holonp::RFFT / holonp::FFT along one axis (rfft.cu:148-297, fft.cu:234-386) for a
+ * contiguous tensor viewed as [outer][n][inner] with the transform over n.  in_dtype:
+ * 1 = U16, 2 = F32, 3 = CF32.  Outputs CF32 [outer][k_end-k_begin][inner] holding bins
+ * [k_begin, k_end) times `scale` (1 for norm "backward", 1/n "forward", 1/sqrt(n) "ortho"). */
 hfx_status hfx_dft_axis(const void *in, int32_t in_dtype, int64_t outer, int32_t n, int64_t inner,
-                        int32_t k_begin, int32_t k_end, void *out, hfx_stream_t stream);
+                        int32_t k_begin, int32_t k_end, float scale, void *out, hfx_stream_t stream);
 
 /* holotask::syncs::FresnelDiffraction (fresnel_diffraction.cu:386-668) over a contiguous
  * [batch][H][W] CF32 (in_real = 0) or F32 (in_real = 1) input, axes (-2, -1). */
@@ -153,11 +153,28 @@ hfx_status hfx_asm_destroy(hfx_asm *a);
 
 /* holotask::syncs::MeanAbs (mean_abs.cu:231-276) / holonp::Mean (mean.cu:254-306) /
  * Multiply+Mean (multiply.cu:69-92) over the plane axis of [planes][npix]:
- *   out[p] = (1/planes) * sum_k w_k * m_k(p),  m = |x| (complex in) or x (real in),
- * with w_k = weights[k] (device array) or 1 when weights is NULL.  power2: m = |x|^2. */
+ *   out[p] = (1/planes) * sum_k w_k * m_k(p),  m = |x| for complex input, and x (or |x|
+ * with HFX_REDUCE_ABS, as MeanAbs does) for real input; w_k = weights[k] (device array) or
+ * 1 when weights is NULL.  HFX_REDUCE_POWER2: m = |x|^2. */
+enum hfx_reduce_flags { HFX_REDUCE_POWER2 = 1u << 0, HFX_REDUCE_ABS = 1u << 1 };
 hfx_status hfx_weighted_mean(const void *in, int32_t in_complex, int64_t planes, int64_t npix,
-                             const float *weights, int32_t power2, float *out,
-                             hfx_stream_t stream);
+                             const float *weights, uint32_t flags, float *out, hfx_stream_t stream);
+
+/* holonp::Copy (copy.cu:40,117-125): gathers the strided view (shape, byte strides, byte
+ * offset from `in`) into `out`, element t of the view (row-major order) going to
+ * out[(t / out_block) * out_pitch + t % out_block] (in elements; out_block = 0 writes
+ * contiguously).  One call also places one input of holonp::Concatenate
+ * (concatenate.cu:210,344-352) in its band of the output. */
+hfx_status hfx_strided_copy(const void *in, const int64_t *shape, const int64_t *in_strides_bytes,
+                            int64_t in_offset_bytes, int32_t ndim, int32_t elem_size, void *out,
+                            int64_t out_block, int64_t out_pitch, hfx_stream_t stream);
+
+/* holonp::Multiply with NumPy broadcasting (multiply.cu:69-92): out (contiguous, shape
+ * out_shape) = a * b, operands given by element strides over out_shape (0 on broadcast
+ * axes); a real operand times a complex one is promoted to x + 0i. */
+hfx_status hfx_multiply(const void *a, int32_t a_complex, const int64_t *a_strides, const void *b,
+                        int32_t b_complex, const int64_t *b_strides, int32_t ndim,
+                        const int64_t *out_shape, void *out, hfx_stream_t stream);
 
 /* holonp::FFTShift (fftshift.cu:110-145,254-260): out[i] = in[(i + n/2) mod n] on every
  * axis whose axes_mask entry is non-zero; elements of elem_size bytes. */

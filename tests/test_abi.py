@@ -1,4 +1,4 @@
-"""The C-ABI library loads without a GPU and exports every symbol include/hfx.h declares.
+"""The C-ABI library loads without a GPU and exports every symbol include/*.h declares.
 
 No compute is called here (this runs on CPU-only machines)."""
 import ctypes
@@ -10,11 +10,17 @@ from holoflow_amd import _lib
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
+HEADERS = ("hfx.h", "hfx_nodes.h")
+
+
 def _declared():
-    with open(os.path.join(ROOT, "include", "hfx.h")) as f:
-        text = f.read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)   # declarations only, not comments
-    return set(re.findall(r"\b(hfx_[a-z0-9_]+)\s*\(", text))
+    names = set()
+    for h in HEADERS:
+        with open(os.path.join(ROOT, "include", h)) as f:
+            text = f.read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)   # declarations only, not comments
+        names |= set(re.findall(r"\b(hfx_[a-z0-9_]+)\s*\(", text))
+    return names
 
 
 def test_library_exports_every_declared_symbol():
@@ -32,7 +38,9 @@ def test_python_binding_covers_the_header():
 def test_binding_loads_and_reports_version():
     L = _lib.lib()
     assert L.hfx_abi_version() == 1
-    assert L.hfx_last_error() == b""
+    # an invalid call (no device work) reports its reason through hfx_last_error
+    assert L.hfx_plan_create(None, None, None) == _lib.HFX_E_INVALID
+    assert L.hfx_last_error() == b"null argument"
 
 
 def test_descriptor_defaults_are_reference_compatible():
