@@ -10,7 +10,8 @@ second over the timed steps.
 
 Also reported on the same line: the mixed-precision MFMA time-DFT variants (throughput and
 relative L2 of every moment image against the FP64 oracle), per-window GPU latency
-percentiles, the roofline of the dominant kernel (the time DFT), and the CPU baseline
+percentiles, the roofline of the dominant kernel (the time DFT fused with the row FFT) and
+of the second kernel (column FFT + moments), the copy ceiling, and the CPU baseline
 (the oracle's FP32 torch-CPU/MKL restatement on a bounded sample of windows).
 """
 from __future__ import annotations
@@ -29,6 +30,17 @@ T, S = 64, 16
 FS = 37e3
 N_FRAMES = 4096
 HBM_PEAK_GBS = 8000.0
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the same counters over a copy of known size)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(kernel, {}).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
 
 
 def log(msg):
@@ -72,8 +84,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note "
-                                   "only (see DESIGN.md), so --gpus must be 1"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU only: the path shards by stream, "
+                                   "window or band bin, which is a design note here, so --gpus must be 1"}))
         sys.exit(2)
 
     import numpy as np
@@ -101,39 +113,61 @@ def main():
     ms_per_step = dt / args.steps * 1e3
     log(f"fp32: {value:.1f} moment frames/s, {ms_per_step:.2f} ms/step")
 
-    # ---- dominant kernel (time DFT) timed with HIP events on the launch stream
-    planes = torch.empty((plan.n_planes, H, W), dtype=torch.complex64, device="cuda")
-    from holoflow_amd._lib import check, lib
-    import ctypes as C
-
+    # ---- the two kernels of a window, each timed alone with HIP events on its stream
     stream = torch.cuda.current_stream()
-    sp = C.c_void_p(stream.cuda_stream)
+    one = plan.new_output()
 
-    def launch_dft(first):
-        check(lib().hfx_plan_time_dft(plan._h, C.c_void_p(ring.data_ptr()), ring.shape[0], first,
-                                      C.c_void_p(planes.data_ptr()), sp))
+    def time_stage(stage, loops):
+        for i in range(3):
+            plan.run_stage(0, ring, i * S, stream=stream)
+            plan.run_stage(1, None, 0, one, stream=stream)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for i in range(loops):
+            if stage == 0:
+                plan.run_stage(0, ring, (i % n_windows) * S, stream=stream)
+            else:
+                plan.run_stage(1, None, 0, one, stream=stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / loops
 
-    for i in range(5):
-        launch_dft(i * S)
+    front_ms = time_stage(0, args.kernel_loop)
+    back_ms = time_stage(1, args.kernel_loop)
+    # algorithmic bytes: front reads the u16 window and writes the band planes once;
+    # back reads the planes once and writes the three moment images
+    front_bytes = T * npix * 2 + nb * npix * 8
+    back_bytes = nb * npix * 8 + 3 * npix * 4
+    front_gbs = front_bytes / (front_ms * 1e-3) / 1e9
+    back_gbs = back_bytes / (back_ms * 1e-3) / 1e9
+    log(f"front kernel (time DFT + row FFT): {front_ms * 1e3:.1f} us, {front_gbs:.0f} GB/s; "
+        f"back kernel (column FFT + moments): {back_ms * 1e3:.1f} us, {back_gbs:.0f} GB/s")
+
+    # ---- copy ceiling on this GPU: the library's copy kernel over 1 GiB, device to device
+    cp_n = 1 << 28
+    src = torch.empty(cp_n, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    cpy = lambda: hfx.copy_contiguous(src, dst, stream=stream)  # noqa: E731
+    for _ in range(3):
+        cpy()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
-    for i in range(args.kernel_loop):
-        launch_dft((i % n_windows) * S)
+    for _ in range(20):
+        cpy()
     e1.record(stream)
     torch.cuda.synchronize()
-    dft_ms = e0.elapsed_time(e1) / args.kernel_loop
-    dft_bytes = T * npix * 2 + nb * npix * 8
-    dft_gbs = dft_bytes / (dft_ms * 1e-3) / 1e9
-    log(f"time-DFT kernel: {dft_ms * 1e3:.1f} us/launch, {dft_gbs:.0f} GB/s")
+    copy_gbs = 2 * cp_n * 4 / (e0.elapsed_time(e1) / 20 * 1e-3) / 1e9
+    del src, dst
+    log(f"copy kernel: {copy_gbs:.0f} GB/s (read + write)")
 
     path_bytes = T * npix * 2 + 2 * nb * npix * 8 + 3 * npix * 4
     path_gbs = path_bytes * value / 1e9
+    traffic = measured_traffic("k_dft_row")
 
     # ---- per-window GPU latency (events around each window's kernels)
     lat = []
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(n_windows)]
-    one = plan.new_output()
     for w in range(n_windows):
         evs[w][0].record(stream)
         plan.moments(ring, w * S, one)
@@ -171,19 +205,21 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = torch.get_num_threads()
-        max_w = 16
-        host = ring[:T + S * (max_w - 1)].cpu().numpy().view(np.uint16)
+        n_pos = min(64, n_windows)
+        host = ring[:T + S * (n_pos - 1)].cpu().numpy().view(np.uint16)
         O.doppler_moments(host[:T], p_or, "f32")        # warm-up
         t0 = time.perf_counter()
         done = 0
-        while done < max_w and (done < 2 or time.perf_counter() - t0 < args.cpu_seconds):
-            O.doppler_moments(host[done * S:done * S + T], p_or, "f32")
+        while done < 2 or time.perf_counter() - t0 < args.cpu_seconds:
+            w = done % n_pos
+            O.doppler_moments(host[w * S:w * S + T], p_or, "f32")
             done += 1
         cdt = time.perf_counter() - t0
         cpu = {"value": round(done / cdt, 4), "unit": "moment frames/s", "cores": threads,
                "kind": "port",
-               "sample": f"{done} consecutive windows (T=64, stride 16) of the same 1024x1024 stream, "
-                         f"FP32 torch-CPU (MKL) restatement, {cdt:.1f} s"}
+               "sample": f"{done} windows (T=64, stride 16) cycling over the first {n_pos} window positions "
+                         f"of the same 1024x1024 stream; FP32 torch-CPU (MKL FFT) restatement of the "
+                         f"reference graph, {threads} threads, {cdt:.1f} s"}
         log(f"cpu baseline: {cpu}")
 
     line = {
@@ -202,10 +238,15 @@ def main():
         "config": {"workload": "1024x1024 u16 12-bit, 4096-frame stream, Fresnel z=0.45 m, "
                                "T=64 stride 16 (253 windows/step), bins [11,26), M0/M1/M2 + fftshift",
                    "frames": args.frames, "windows_per_step": n_windows, "precision": "fp32"},
-        "roofline": {"bound": "hbm", "achieved": round(dft_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(dft_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "time DFT (k_dft_f32)", "kernel_us": round(dft_ms * 1e3, 2),
-                     "algorithmic_bytes_per_launch": dft_bytes},
+        "roofline": {"bound": "hbm", "achieved": round(front_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(front_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_dft_row<1024,2,0,false> (time DFT fused with the row FFT)",
+                     "kernel_us": round(front_ms * 1e3, 2), "algorithmic_bytes_per_launch": front_bytes},
+        "kernels": {"front_us": round(front_ms * 1e3, 2), "front_gbs": round(front_gbs, 1),
+                    "back_us": round(back_ms * 1e3, 2), "back_gbs": round(back_gbs, 1),
+                    "back_kernel": "k_wcol<1024,1> (column FFT + |.| + moments)",
+                    "back_algorithmic_bytes_per_launch": back_bytes,
+                    "copy_ceiling_gbs": round(copy_gbs, 1)},
         "path_roofline": {"achieved": round(path_gbs, 1), "frac": round(path_gbs / HBM_PEAK_GBS, 4),
                           "algorithmic_bytes_per_window": path_bytes},
         "latency_ms": {"window_gpu_p50": round(float(np.percentile(lat, 50)), 4),

@@ -115,6 +115,7 @@ SIGNATURES = {
     "hfx_moments": (C.c_int, [_VP, _VP, C.c_int64, C.c_int64, _VP, _VP]),
     "hfx_moments_batch": (C.c_int, [_VP, _VP, C.c_int64, C.c_int64, C.c_int32, C.c_int64, _VP, _VP]),
     "hfx_plan_time_dft": (C.c_int, [_VP, _VP, C.c_int64, C.c_int64, _VP, _VP]),
+    "hfx_plan_run_stage": (C.c_int, [_VP, C.c_int32, _VP, C.c_int64, C.c_int64, _VP, _VP]),
     "hfx_dft_axis": (C.c_int, [_VP, C.c_int32, C.c_int64, C.c_int32, C.c_int64, C.c_int32, C.c_int32,
                                C.c_float, _VP, _VP]),
     "hfx_fresnel_create": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,

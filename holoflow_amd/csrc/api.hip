@@ -328,15 +328,35 @@ static hfx::PassArgs base_pass(int H, int W, int planes, int blk_c) {
   return a;
 }
 
-static hfx_status window_moments(hfx_plan *p, const uint16_t *ring, int64_t ring_frames,
-                                 int64_t first, float *out, hipStream_t st) {
+// Front of the schedule: the window's band planes, row-transformed, in the plan's blocked
+// workspace (one fused kernel when the frame width allows, else time DFT + row pass).
+static hfx_status window_front(hfx_plan *p, const uint16_t *ring, int64_t ring_frames, int64_t first,
+                               hipStream_t st) {
   if (p->fused_row) {
     hfx::DftLaunch L = dft_launch(p, ring, ring_frames, first, p->planes_blk);
     if (hfx::launch_time_dft_row(L, p->OH, p->blk_c, p->n_planes, p->tw_w, st) != 0)
       return fail(HFX_E_HIP, "fused time-DFT/row-FFT launch failed");
-  } else {
-    HFX_TRY(run_dft(p, ring, ring_frames, first, p->planes_nat, st));
+    return HFX_OK;
   }
+  HFX_TRY(run_dft(p, ring, ring_frames, first, p->planes_nat, st));
+  // row pass: natural planes (lens already applied for Fresnel) -> blocked spectrum rows
+  hfx::PassArgs r = base_pass(p->OH, p->OW, p->n_planes, p->blk_c);
+  r.in = p->planes_nat;
+  r.in_mode = hfx::IN_NAT_C;
+  r.in_h = p->H;
+  r.in_w = p->W;
+  r.off_y = (p->OH - p->H) / 2;
+  r.off_x = (p->OW - p->W) / 2;
+  r.out = p->planes_blk;
+  r.out_mode = hfx::OUT_BLK;
+  r.tw = p->tw_w;
+  if (hfx::launch_row_pass(r, st) != 0) return fail(HFX_E_HIP, "row pass launch failed");
+  return HFX_OK;
+}
+
+// Back of the schedule: column pass (Fresnel: with |.| and the moment reduction; ASM: with
+// the transfer function, then the inverse row pass with the reduction) into `out`.
+static hfx_status window_back(hfx_plan *p, float *out, hipStream_t st) {
   const long long on = p->onpix;
   hfx::MomentArgs mom{};
   for (int i = 0; i < p->n_out; ++i) mom.out[i] = out + i * on;
@@ -345,21 +365,6 @@ static hfx_status window_moments(hfx_plan *p, const uint16_t *ring, int64_t ring
   mom.split = p->d.split_bin > 0 ? p->d.split_bin - p->d.bin_begin : 0;
   mom.power2 = (p->d.flags & HFX_FLAG_POWER2) ? 1 : 0;
   mom.shift = (p->d.flags & HFX_FLAG_FFTSHIFT) ? 1 : 0;
-
-  if (!p->fused_row) {
-    // row pass: natural planes (lens already applied for Fresnel) -> blocked spectrum rows
-    hfx::PassArgs r = base_pass(p->OH, p->OW, p->n_planes, p->blk_c);
-    r.in = p->planes_nat;
-    r.in_mode = hfx::IN_NAT_C;
-    r.in_h = p->H;
-    r.in_w = p->W;
-    r.off_y = (p->OH - p->H) / 2;
-    r.off_x = (p->OW - p->W) / 2;
-    r.out = p->planes_blk;
-    r.out_mode = hfx::OUT_BLK;
-    r.tw = p->tw_w;
-    if (hfx::launch_row_pass(r, st) != 0) return fail(HFX_E_HIP, "row pass launch failed");
-  }
 
   hfx::PassArgs c = base_pass(p->OH, p->OW, p->n_planes, p->blk_c);
   c.in = p->planes_blk;
@@ -384,6 +389,26 @@ static hfx_status window_moments(hfx_plan *p, const uint16_t *ring, int64_t ring
     if (hfx::launch_row_pass(r2, st) != 0) return fail(HFX_E_HIP, "inverse row pass launch failed");
   }
   return HFX_OK;
+}
+
+static hfx_status window_moments(hfx_plan *p, const uint16_t *ring, int64_t ring_frames,
+                                 int64_t first, float *out, hipStream_t st) {
+  HFX_TRY(window_front(p, ring, ring_frames, first, st));
+  return window_back(p, out, st);
+}
+
+hfx_status hfx_plan_run_stage(hfx_plan *p, int32_t stage, const uint16_t *ring, int64_t ring_frames,
+                              int64_t first_frame, float *out, hfx_stream_t stream) {
+  if (!p) return fail(HFX_E_INVALID, "null plan");
+  if (stage == 0) {
+    HFX_TRY(check_ring(p, ring, ring_frames, first_frame));
+    return window_front(p, ring, ring_frames, first_frame, S(stream));
+  }
+  if (stage == 1) {
+    if (!out) return fail(HFX_E_INVALID, "null output");
+    return window_back(p, out, S(stream));
+  }
+  return fail(HFX_E_INVALID, "stage must be 0 (front) or 1 (back)");
 }
 
 hfx_status hfx_moments(hfx_plan *p, const uint16_t *ring, int64_t ring_frames,

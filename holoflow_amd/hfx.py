@@ -120,6 +120,14 @@ class MomentPlan:
                                       stride, _ptr(out), _stream(stream)))
         return out
 
+    def run_stage(self, stage: int, ring: Optional[torch.Tensor], first_frame: int = 0,
+                  out: Optional[torch.Tensor] = None, stream=None) -> None:
+        """Stage 0 (time DFT + row FFT into the workspace) or stage 1 (rest -> out)."""
+        rp = _ptr(ring) if ring is not None else None
+        op = _ptr(out) if out is not None else None
+        n = ring.shape[0] if ring is not None else 0
+        check(lib().hfx_plan_run_stage(self._h, stage, rp, n, first_frame, op, _stream(stream)))
+
     def time_dft(self, ring: torch.Tensor, first_frame: int = 0, stream=None) -> torch.Tensor:
         """Runs the time-DFT stage only: [n_planes][H][W] complex64 (lens applied for Fresnel)."""
         out = torch.empty((self.n_planes, self.cfg.height, self.cfg.width), dtype=torch.complex64,
@@ -232,6 +240,18 @@ def fftshift(x: torch.Tensor, axes: Sequence[int], stream=None) -> torch.Tensor:
     out = torch.empty_like(x)
     check(lib().hfx_fftshift(_ptr(x), _ptr(out), x.element_size(), nd, shape, mask, _stream(stream)))
     return out
+
+
+def copy_contiguous(src: torch.Tensor, dst: torch.Tensor, stream=None) -> torch.Tensor:
+    """Device copy of a contiguous tensor through the library's gather kernel."""
+    if not (src.is_contiguous() and dst.is_contiguous()) or src.nbytes != dst.nbytes:
+        raise ValueError("copy_contiguous needs two contiguous tensors of the same size")
+    e = 4 if src.nbytes % 4 == 0 else 1
+    n = src.nbytes // e
+    shape = (C.c_int64 * 1)(n)
+    strides = (C.c_int64 * 1)(e)
+    check(lib().hfx_strided_copy(_ptr(src), shape, strides, 0, 1, e, _ptr(dst), 0, 0, _stream(stream)))
+    return dst
 
 
 def convert_u16(x: torch.Tensor, to_complex=False, stream=None) -> torch.Tensor:

@@ -112,6 +112,13 @@ hfx_status hfx_moments_batch(hfx_plan *p, const uint16_t *ring, int64_t ring_fra
  * to planes_out (device). */
 hfx_status hfx_plan_time_dft(hfx_plan *p, const uint16_t *ring, int64_t ring_frames,
                              int64_t first_frame, void *planes_out, hfx_stream_t stream);
+/* One half of hfx_moments, so that each kernel of the schedule can be timed alone:
+ * stage 0 (front) = time DFT of the window and the forward row FFT (one fused kernel when
+ * the width allows) into the plan's workspace; stage 1 (back) = the remaining passes and
+ * the moment reduction from that workspace into out.  Stage 0 then stage 1 on one stream
+ * is exactly hfx_moments(plan, ring, ring_frames, first_frame, out). */
+hfx_status hfx_plan_run_stage(hfx_plan *p, int32_t stage, const uint16_t *ring, int64_t ring_frames,
+                              int64_t first_frame, float *out, hfx_stream_t stream);
 
 /* ------------------------------------------------------------------------------------
  * Per-node kinds (drop-in replacements of individual reference tasks).

@@ -109,6 +109,24 @@ def test_batch_matches_single_windows(gpu):
         assert torch.equal(batch[w], single)
 
 
+@pytest.mark.parametrize("prop,W", [("fresnel", 512), ("asm", 64)])
+def test_stages_compose_to_moments(gpu, prop, W):
+    # stage 0 then stage 1 is hfx_moments (fused front kernel at 512, unfused at 64)
+    from holoflow_amd import hfx
+
+    p = O.PathParams(64, W, 32, 8, 6, 13, prop, 852e-9, 12e-6, 12e-6, 0.3)
+    ring = _frames(64, W, 48)
+    plan = hfx.MomentPlan(_cfg(p))
+    ref = plan.moments(ring, 8)
+    out = plan.new_output()
+    plan.run_stage(0, ring, 8)
+    plan.run_stage(1, None, 0, out)
+    assert torch.equal(out, ref)
+    from holoflow_amd._lib import HfxInvalidArgument
+    with pytest.raises(HfxInvalidArgument):
+        plan.run_stage(2, ring, 0, out)
+
+
 def test_update_z_matches_fresh_plan(gpu):
     from holoflow_amd import hfx
 
