@@ -70,6 +70,34 @@ def time_batch(plan, ring, n_windows, out, steps, warmup, torch):
     return time.perf_counter() - t0
 
 
+def serving_leg(args, np, torch, load=0.7, slots=16):
+    """Mixed-stream load: sizes {512^2, 768^2, 1024^2, 2048x512}, T in {32, 64, 128}, S = T/4,
+    z ~ U(0.2, 0.6) m, f_s ~ U(20, 75) kHz, ASM.  Each stream's windows arrive as a Poisson
+    process whose rate is set so that `slots` concurrent streams offer `load` of the GPU
+    (from each stream's own calibrated window cost); streams arrive as a Poisson process
+    too and wait for a slot and for memory."""
+    from holoflow_amd import serving
+
+    specs = serving.mixed_streams(args.serving_streams, seed=2024)
+    costs = np.array([serving.window_cost_us(s) for s in specs])
+    rates = load / (slots * costs * 1e-6)
+    life = args.serving_windows / rates
+    rng = np.random.default_rng(7)
+    gaps = rng.exponential(float(np.mean(life)) / slots, len(specs))
+    arrivals = np.concatenate([[0.0], np.cumsum(gaps[1:])])
+    res = serving.run_workload(specs, arrivals, rates, args.serving_windows,
+                               server=serving.server_desc(max_streams=slots, n_lanes=4), seed=11,
+                               time_limit_s=240.0)
+    summary = serving.summarize(specs, res)
+    summary.update({
+        "offered_gpu_load_at_16_streams": load,
+        "window_cost_us_alone_p50": round(float(np.median(costs)), 1),
+        "note": "latency = ingest of the window's last frame -> moments complete (HIP events); "
+                "window period = stride / f_s of the stream",
+    })
+    return summary
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +107,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-streams", type=int, default=24)
+    ap.add_argument("--serving-windows", type=int, default=128)
     ap.add_argument("--kernel-loop", type=int, default=200,
                     help="launches of the dominant kernel timed with HIP events")
     args = ap.parse_args()
@@ -201,6 +232,13 @@ def main():
             mp.close()
         mixed["fp32_rel_l2_vs_fp64"] = {k: float(f"{fp32_err[i]:.3e}") for i, k in enumerate(("M0", "M1", "M2"))}
 
+    # ---- serving: Poisson arrivals of mixed acquisition streams, 16 slots, native loop
+    serve = None
+    if not args.no_serving:
+        del out
+        serve = serving_leg(args, np, torch)
+        log(f"serving: {serve}")
+
     # ---- CPU baseline: the oracle's FP32 restatement (torch-CPU, MKL FFTs), host cores
     cpu = None
     if not args.no_cpu:
@@ -253,6 +291,7 @@ def main():
                        "window_gpu_p99": round(float(np.percentile(lat, 99)), 4),
                        "window_period": round(window_period_ms, 4)},
         "mixed": mixed,
+        "serving": serve,
         "cpu_baseline": cpu,
     }
     if cpu:

@@ -148,6 +148,58 @@ SIGNATURES = {
     "hfx_node_destroy": (C.c_int, [_VP]),
 }
 
+
+
+class ServerDesc(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_int64), ("hbm_reserve_bytes", C.c_int64),
+                ("max_streams", C.c_int32), ("n_lanes", C.c_int32), ("use_graphs", C.c_int32)]
+
+
+class StreamDesc(C.Structure):
+    _fields_ = [("plan", PlanDesc), ("queue_windows", C.c_int32), ("camera", SynthDesc)]
+
+
+class WindowDone(C.Structure):
+    _fields_ = [("stream_id", C.c_int32), ("n_outputs", C.c_int32), ("window", C.c_int64),
+                ("latency_us", C.c_float), ("gpu_us", C.c_float)]
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [("active_streams", C.c_int32), ("max_streams", C.c_int32), ("bytes_held", C.c_int64),
+                ("bytes_free", C.c_int64), ("windows_launched", C.c_int64), ("windows_retired", C.c_int64)]
+
+
+class WorkloadDesc(C.Structure):
+    _fields_ = [("server", ServerDesc), ("n_streams", C.c_int32), ("streams", C.POINTER(StreamDesc)),
+                ("arrival_s", C.POINTER(C.c_double)), ("window_rate_hz", C.POINTER(C.c_double)),
+                ("windows_per_stream", C.c_int32), ("seed", C.c_uint64), ("capture_stream", C.c_int32),
+                ("capture_out", C.POINTER(C.c_float)), ("time_limit_s", C.c_double)]
+
+
+class WorkloadResult(C.Structure):
+    _fields_ = [("wall_s", C.c_double), ("windows_retired", C.c_int64), ("max_concurrent", C.c_int32),
+                ("admissions_deferred", C.c_int32), ("ingest_deferred", C.c_int64),
+                ("latency_us", C.POINTER(C.c_float)), ("gpu_us", C.POINTER(C.c_float)),
+                ("admit_s", C.POINTER(C.c_double)), ("finish_s", C.POINTER(C.c_double))]
+
+
+SIGNATURES.update({
+    # include/hfx_serve.h
+    "hfx_server_desc_init": (None, [C.POINTER(ServerDesc)]),
+    "hfx_stream_desc_init": (None, [C.POINTER(StreamDesc), C.c_int32, C.c_int32]),
+    "hfx_server_stream_bytes": (C.c_int, [C.POINTER(StreamDesc), C.POINTER(C.c_int64)]),
+    "hfx_server_create": (C.c_int, [C.POINTER(ServerDesc), C.POINTER(C.c_void_p)]),
+    "hfx_server_destroy": (C.c_int, [C.c_void_p]),
+    "hfx_server_admit": (C.c_int, [C.c_void_p, C.POINTER(StreamDesc), C.POINTER(C.c_int32)]),
+    "hfx_server_ingest": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64]),
+    "hfx_server_schedule": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    "hfx_server_poll": (C.c_int, [C.c_void_p, C.POINTER(WindowDone), C.c_int32, C.POINTER(C.c_int32)]),
+    "hfx_server_window_output": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.POINTER(C.c_void_p)]),
+    "hfx_server_release": (C.c_int, [C.c_void_p, C.c_int32]),
+    "hfx_server_info_get": (C.c_int, [C.c_void_p, C.POINTER(ServerInfo)]),
+    "hfx_serve_workload": (C.c_int, [C.POINTER(WorkloadDesc), C.POINTER(WorkloadResult)]),
+})
+
 _lib = None
 _lock = threading.Lock()
 

@@ -1,0 +1,514 @@
+// Serving loop for concurrent acquisition streams on one GPU (include/hfx_serve.h).
+//
+// Per admitted stream the device holds: a frame ring of R = T + q*S frames (q = queued
+// windows), a source pool of P frames the synthetic camera plays from (generated once at
+// admission by hfx_synth_frames, so ingest is a device-to-device copy, as a frame grabber's
+// DMA would be), the moment plan (tables + workspace) and one output slot per window
+// position (n_pos = R / S positions).  Window w reads ring frames [w*S, w*S + T) mod R,
+// which for R = n_pos*S is position p = w mod n_pos starting at p*S: every position is
+// captured once into a hipGraph (front + back kernels of hfx_moments) and replayed.
+//
+// Ordering: the camera copies on one ingest stream and records `ready[p]` after the last
+// frame of window w; the window's lane waits on it, records `launch[p]`, replays the graph
+// and records `done[p]`.  Ingest refuses (HFX_NOT_READY) to overwrite frames that a window
+// not yet retired reads, and a window is retired only after its done event completed, so
+// no frame is overwritten while read and no event is re-recorded while still needed.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "hfx.h"
+#include "hfx_serve.h"
+
+namespace hfx {
+void set_last_error(const std::string &msg);  // libhfx's thread-local last error (api.hip)
+}
+
+namespace {
+
+hfx_status fail(hfx_status code, const std::string &msg) {
+  hfx::set_last_error(msg);
+  return code;
+}
+
+#define SRV_HIP(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(HFX_E_HIP, std::string(#expr " failed: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+#define SRV_TRY(expr)            \
+  do {                           \
+    hfx_status s_ = (expr);      \
+    if (s_ != HFX_OK) return s_; \
+  } while (0)
+
+struct Geometry {
+  int64_t T, S, R, n_pos, P, npix, out_elems, n_planes, oh, ow, n_out;
+  int64_t ring_bytes, pool_bytes, out_bytes, plan_bytes;
+  int64_t total() const { return ring_bytes + pool_bytes + out_bytes + plan_bytes + npix * 8; }
+};
+
+hfx_status geometry(const hfx_stream_desc &d, Geometry &g) {
+  const hfx_plan_desc &p = d.plan;
+  if (p.window <= 0 || p.stride <= 0 || p.height <= 0 || p.width <= 0)
+    return fail(HFX_E_INVALID, "stream: frame size, window and stride must be positive");
+  if (p.window % p.stride != 0) return fail(HFX_E_INVALID, "stream: window must be a multiple of stride");
+  if (d.queue_windows < 1) return fail(HFX_E_INVALID, "stream: queue_windows must be >= 1");
+  if (d.camera.height != p.height || d.camera.width != p.width)
+    return fail(HFX_E_INVALID, "stream: camera frame size differs from the plan's");
+  g.T = p.window;
+  g.S = p.stride;
+  g.n_pos = g.T / g.S + d.queue_windows;
+  g.R = g.n_pos * g.S;
+  g.P = 4 * g.T;
+  g.npix = (int64_t)p.height * p.width;
+  g.oh = p.pad_height > 0 ? p.pad_height : p.height;
+  g.ow = p.pad_width > 0 ? p.pad_width : p.width;
+  g.n_out = p.split_bin > 0 ? 5 : 3;
+  g.out_elems = g.n_out * g.oh * g.ow;
+  const int64_t nb = (int64_t)p.bin_end - p.bin_begin;
+  g.n_planes = (p.flags & HFX_FLAG_BOTH_SIDEBANDS) ? 2 * nb : nb;
+  g.ring_bytes = g.R * g.npix * 2;
+  g.pool_bytes = g.P * g.npix * 2;
+  g.out_bytes = g.n_pos * g.out_elems * 4;
+  // plan workspace upper bound: blocked + natural band planes, transfer table, small tables
+  g.plan_bytes = 2 * g.n_planes * g.oh * g.ow * 8 + g.oh * g.ow * 8 + (1 << 20);
+  return HFX_OK;
+}
+
+struct Stream {
+  int32_t id = -1;
+  hfx_stream_desc d{};
+  Geometry g{};
+  int lane = 0;
+  hfx_plan *plan = nullptr;
+  uint16_t *ring = nullptr, *pool = nullptr;
+  float *outs = nullptr;
+  double *phase = nullptr;
+  std::vector<hipGraphExec_t> graphs;
+  std::vector<hipEvent_t> ev_ready, ev_launch, ev_done;
+  int64_t frames_in = 0, launched = 0, retired = 0;
+
+  int64_t ready_windows() const { return frames_in >= g.T ? (frames_in - g.T) / g.S + 1 : 0; }
+
+  void free_all() {
+    for (auto &x : graphs)
+      if (x) (void)hipGraphExecDestroy(x);
+    for (auto *v : {&ev_ready, &ev_launch, &ev_done})
+      for (auto &e : *v)
+        if (e) (void)hipEventDestroy(e);
+    graphs.clear();
+    ev_ready.clear();
+    ev_launch.clear();
+    ev_done.clear();
+    if (plan) hfx_plan_destroy(plan);
+    plan = nullptr;
+    for (void *p : {(void *)ring, (void *)pool, (void *)outs, (void *)phase})
+      if (p) (void)hipFree(p);
+    ring = pool = nullptr;
+    outs = nullptr;
+    phase = nullptr;
+  }
+};
+
+}  // namespace
+
+struct hfx_server {
+  hfx_server_desc d{};
+  int device = 0;
+  hipStream_t ingest = nullptr, capture = nullptr;
+  std::vector<hipStream_t> lanes;
+  std::vector<std::unique_ptr<Stream>> slots;
+  std::vector<int> lane_load;
+  int64_t bytes_held = 0, windows_launched = 0, windows_retired = 0;
+};
+
+namespace {
+
+Stream *stream_of(hfx_server *s, int32_t id) {
+  if (!s || id < 0 || id >= (int32_t)s->slots.size()) return nullptr;
+  return s->slots[id].get();
+}
+
+hfx_status copy_frames(hfx_server *s, Stream &st, int64_t first, int64_t n) {
+  // frame f of the stream = pool frame f mod P, written to ring slot f mod R
+  const Geometry &g = st.g;
+  const size_t fb = (size_t)g.npix * 2;
+  int64_t f = first;
+  while (f < first + n) {
+    const int64_t rs = f % g.R, ps = f % g.P;
+    const int64_t len = std::min({first + n - f, g.R - rs, g.P - ps});
+    SRV_HIP(hipMemcpyAsync(st.ring + rs * g.npix, st.pool + ps * g.npix, (size_t)len * fb,
+                           hipMemcpyDeviceToDevice, s->ingest));
+    f += len;
+  }
+  return HFX_OK;
+}
+
+hfx_status build_stream(hfx_server *s, Stream &st) {
+  const Geometry &g = st.g;
+  hipStream_t cs = s->capture;
+  SRV_TRY(hfx_plan_create(&st.d.plan, cs, &st.plan));
+  SRV_HIP(hipMalloc((void **)&st.ring, (size_t)g.ring_bytes));
+  SRV_HIP(hipMalloc((void **)&st.pool, (size_t)g.pool_bytes));
+  SRV_HIP(hipMalloc((void **)&st.outs, (size_t)g.out_bytes));
+  SRV_HIP(hipMalloc((void **)&st.phase, (size_t)g.npix * 8));
+  SRV_HIP(hipMemsetAsync(st.phase, 0, (size_t)g.npix * 8, cs));
+  SRV_TRY(hfx_synth_frames(&st.d.camera, st.pool, 0, g.P, st.phase, cs));
+  SRV_HIP(hipStreamSynchronize(cs));
+  st.ev_ready.assign(g.n_pos, nullptr);
+  st.ev_launch.assign(g.n_pos, nullptr);
+  st.ev_done.assign(g.n_pos, nullptr);
+  for (int64_t p = 0; p < g.n_pos; ++p) {
+    SRV_HIP(hipEventCreate(&st.ev_ready[p]));
+    SRV_HIP(hipEventCreate(&st.ev_launch[p]));
+    SRV_HIP(hipEventCreate(&st.ev_done[p]));
+  }
+  if (s->d.use_graphs) {
+    st.graphs.assign(g.n_pos, nullptr);
+    for (int64_t p = 0; p < g.n_pos; ++p) {
+      hipGraph_t graph = nullptr;
+      SRV_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      const hfx_status rc = hfx_moments(st.plan, st.ring, g.R, p * g.S, st.outs + p * g.out_elems, cs);
+      const hipError_t e = hipStreamEndCapture(cs, &graph);
+      if (rc != HFX_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      SRV_HIP(e);
+      const hipError_t ie = hipGraphInstantiate(&st.graphs[p], graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      SRV_HIP(ie);
+    }
+  }
+  return HFX_OK;
+}
+
+hfx_status launch_window(hfx_server *s, Stream &st) {
+  const int64_t w = st.launched;
+  const int64_t p = w % st.g.n_pos;
+  hipStream_t lane = s->lanes[st.lane];
+  SRV_HIP(hipStreamWaitEvent(lane, st.ev_ready[p], 0));
+  SRV_HIP(hipEventRecord(st.ev_launch[p], lane));
+  if (s->d.use_graphs) {
+    SRV_HIP(hipGraphLaunch(st.graphs[p], lane));
+  } else {
+    SRV_TRY(hfx_moments(st.plan, st.ring, st.g.R, p * st.g.S, st.outs + p * st.g.out_elems, lane));
+  }
+  SRV_HIP(hipEventRecord(st.ev_done[p], lane));
+  ++st.launched;
+  ++s->windows_launched;
+  return HFX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void hfx_server_desc_init(hfx_server_desc *d) {
+  std::memset(d, 0, sizeof(*d));
+  d->hbm_budget_bytes = 0;
+  d->hbm_reserve_bytes = (int64_t)8 << 30;
+  d->max_streams = 16;
+  d->n_lanes = 4;
+  d->use_graphs = 1;
+}
+
+void hfx_stream_desc_init(hfx_stream_desc *d, int32_t height, int32_t width) {
+  std::memset(d, 0, sizeof(*d));
+  hfx_plan_desc_init(&d->plan);
+  d->plan.height = height;
+  d->plan.width = width;
+  d->queue_windows = 4;
+  hfx_synth_desc_init(&d->camera, height, width);
+}
+
+hfx_status hfx_server_stream_bytes(const hfx_stream_desc *d, int64_t *bytes) {
+  if (!d || !bytes) return fail(HFX_E_INVALID, "null argument");
+  Geometry g;
+  SRV_TRY(geometry(*d, g));
+  *bytes = g.total();
+  return HFX_OK;
+}
+
+hfx_status hfx_server_create(const hfx_server_desc *d, hfx_server **out) {
+  if (!d || !out) return fail(HFX_E_INVALID, "null argument");
+  *out = nullptr;
+  if (d->max_streams < 1 || d->max_streams > 1024) return fail(HFX_E_INVALID, "max_streams must be in [1, 1024]");
+  if (d->n_lanes < 1 || d->n_lanes > 16) return fail(HFX_E_INVALID, "n_lanes must be in [1, 16]");
+  if (d->hbm_budget_bytes < 0 || d->hbm_reserve_bytes < 0) return fail(HFX_E_INVALID, "negative memory size");
+  auto s = std::make_unique<hfx_server>();
+  s->d = *d;
+  SRV_HIP(hipGetDevice(&s->device));
+  SRV_HIP(hipStreamCreateWithFlags(&s->ingest, hipStreamNonBlocking));
+  SRV_HIP(hipStreamCreateWithFlags(&s->capture, hipStreamNonBlocking));
+  s->lanes.assign(d->n_lanes, nullptr);
+  for (auto &l : s->lanes) SRV_HIP(hipStreamCreateWithFlags(&l, hipStreamNonBlocking));
+  s->lane_load.assign(d->n_lanes, 0);
+  s->slots.resize(d->max_streams);
+  *out = s.release();
+  return HFX_OK;
+}
+
+hfx_status hfx_server_destroy(hfx_server *s) {
+  if (!s) return HFX_OK;
+  (void)hipDeviceSynchronize();
+  for (auto &st : s->slots)
+    if (st) st->free_all();
+  for (auto l : s->lanes)
+    if (l) (void)hipStreamDestroy(l);
+  if (s->ingest) (void)hipStreamDestroy(s->ingest);
+  if (s->capture) (void)hipStreamDestroy(s->capture);
+  delete s;
+  return HFX_OK;
+}
+
+hfx_status hfx_server_admit(hfx_server *s, const hfx_stream_desc *d, int32_t *stream_id) {
+  if (!s || !d || !stream_id) return fail(HFX_E_INVALID, "null argument");
+  *stream_id = -1;
+  Geometry g;
+  SRV_TRY(geometry(*d, g));
+  int slot = -1;
+  for (int i = 0; i < (int)s->slots.size(); ++i)
+    if (!s->slots[i]) {
+      slot = i;
+      break;
+    }
+  if (slot < 0) return fail(HFX_NOT_READY, "no free stream slot");
+  size_t free_b = 0, total_b = 0;
+  SRV_HIP(hipMemGetInfo(&free_b, &total_b));
+  const int64_t need = g.total();
+  if (s->d.hbm_budget_bytes > 0) {
+    if (s->bytes_held + need > s->d.hbm_budget_bytes) return fail(HFX_NOT_READY, "stream does not fit the HBM budget now");
+  }
+  if ((int64_t)free_b - need < s->d.hbm_reserve_bytes) return fail(HFX_NOT_READY, "not enough free HBM now");
+  auto st = std::make_unique<Stream>();
+  st->id = slot;
+  st->d = *d;
+  st->g = g;
+  st->lane = (int)(std::min_element(s->lane_load.begin(), s->lane_load.end()) - s->lane_load.begin());
+  const hfx_status rc = build_stream(s, *st);
+  if (rc != HFX_OK) {
+    const std::string msg = hfx_last_error();
+    (void)hipStreamSynchronize(s->capture);
+    st->free_all();
+    return fail(rc, msg);
+  }
+  s->lane_load[st->lane]++;
+  s->bytes_held += need;
+  s->slots[slot] = std::move(st);
+  *stream_id = slot;
+  return HFX_OK;
+}
+
+hfx_status hfx_server_ingest(hfx_server *s, int32_t stream_id, int64_t n_frames) {
+  Stream *st = stream_of(s, stream_id);
+  if (!st) return fail(HFX_E_INVALID, "unknown stream id");
+  const Geometry &g = st->g;
+  if (n_frames < 0 || n_frames > g.R) return fail(HFX_E_INVALID, "n_frames must be in [0, ring frames]");
+  if (n_frames == 0) return HFX_OK;
+  const int64_t F = st->frames_in + n_frames;
+  // windows reading a frame < F - R must be retired before those frames are overwritten
+  const int64_t over = F - g.R;
+  const int64_t w_need = over > 0 ? (over + g.S - 1) / g.S : 0;
+  if (st->retired < w_need) return fail(HFX_NOT_READY, "ring full: windows not retired yet");
+  const int64_t old_ready = st->ready_windows();
+  SRV_TRY(copy_frames(s, *st, st->frames_in, n_frames));
+  st->frames_in = F;
+  for (int64_t w = old_ready; w < st->ready_windows(); ++w)
+    SRV_HIP(hipEventRecord(st->ev_ready[w % g.n_pos], s->ingest));
+  return HFX_OK;
+}
+
+hfx_status hfx_server_schedule(hfx_server *s, int32_t *launched) {
+  if (!s) return fail(HFX_E_INVALID, "null server");
+  int32_t n = 0;
+  for (auto &up : s->slots) {
+    if (!up) continue;
+    Stream &st = *up;
+    while (st.launched < st.ready_windows()) {
+      SRV_TRY(launch_window(s, st));
+      ++n;
+    }
+  }
+  if (launched) *launched = n;
+  return HFX_OK;
+}
+
+hfx_status hfx_server_poll(hfx_server *s, hfx_window_done *out, int32_t cap, int32_t *n) {
+  if (!s || !n || (cap > 0 && !out)) return fail(HFX_E_INVALID, "null argument");
+  int32_t k = 0;
+  for (auto &up : s->slots) {
+    if (!up) continue;
+    Stream &st = *up;
+    while (k < cap && st.retired < st.launched) {
+      const int64_t p = st.retired % st.g.n_pos;
+      const hipError_t q = hipEventQuery(st.ev_done[p]);
+      if (q == hipErrorNotReady) break;
+      SRV_HIP(q);
+      float lat = 0.f, gpu = 0.f;
+      SRV_HIP(hipEventElapsedTime(&lat, st.ev_ready[p], st.ev_done[p]));
+      SRV_HIP(hipEventElapsedTime(&gpu, st.ev_launch[p], st.ev_done[p]));
+      out[k].stream_id = st.id;
+      out[k].n_outputs = (int32_t)st.g.n_out;
+      out[k].window = st.retired;
+      out[k].latency_us = lat * 1e3f;
+      out[k].gpu_us = gpu * 1e3f;
+      ++k;
+      ++st.retired;
+      ++s->windows_retired;
+    }
+  }
+  *n = k;
+  return HFX_OK;
+}
+
+hfx_status hfx_server_window_output(hfx_server *s, int32_t stream_id, int64_t window, const float **out) {
+  Stream *st = stream_of(s, stream_id);
+  if (!st || !out) return fail(HFX_E_INVALID, "unknown stream id or null argument");
+  if (window < 0 || window >= st->retired || window + st->g.n_pos <= st->launched)
+    return fail(HFX_E_INVALID, "window not retired or its output slot was reused");
+  *out = st->outs + (window % st->g.n_pos) * st->g.out_elems;
+  return HFX_OK;
+}
+
+hfx_status hfx_server_release(hfx_server *s, int32_t stream_id) {
+  Stream *st = stream_of(s, stream_id);
+  if (!st) return fail(HFX_E_INVALID, "unknown stream id");
+  SRV_HIP(hipStreamSynchronize(s->ingest));
+  SRV_HIP(hipStreamSynchronize(s->lanes[st->lane]));
+  s->lane_load[st->lane]--;
+  s->bytes_held -= st->g.total();
+  st->free_all();
+  s->slots[stream_id].reset();
+  return HFX_OK;
+}
+
+hfx_status hfx_server_info_get(hfx_server *s, hfx_server_info *info) {
+  if (!s || !info) return fail(HFX_E_INVALID, "null argument");
+  size_t free_b = 0, total_b = 0;
+  SRV_HIP(hipMemGetInfo(&free_b, &total_b));
+  info->active_streams = 0;
+  for (auto &up : s->slots) info->active_streams += up ? 1 : 0;
+  info->max_streams = s->d.max_streams;
+  info->bytes_held = s->bytes_held;
+  info->bytes_free = (int64_t)free_b;
+  info->windows_launched = s->windows_launched;
+  info->windows_retired = s->windows_retired;
+  return HFX_OK;
+}
+
+// ---------------------------------------------------------------------------- workload
+hfx_status hfx_serve_workload(const hfx_workload_desc *d, hfx_workload_result *r) {
+  if (!d || !r || d->n_streams < 1 || !d->streams || !d->arrival_s || !d->window_rate_hz ||
+      d->windows_per_stream < 1)
+    return fail(HFX_E_INVALID, "invalid workload description");
+  if (d->capture_stream >= d->n_streams || (d->capture_stream >= 0 && !d->capture_out))
+    return fail(HFX_E_INVALID, "invalid capture stream");
+  for (int i = 0; i < d->n_streams; ++i)
+    if (!(d->window_rate_hz[i] > 0)) return fail(HFX_E_INVALID, "window rates must be positive");
+  hfx_server *srv = nullptr;
+  SRV_TRY(hfx_server_create(&d->server, &srv));
+  std::unique_ptr<hfx_server, hfx_status (*)(hfx_server *)> guard(srv, hfx_server_destroy);
+
+  struct Live {
+    int32_t id = -1;
+    bool admitted = false, released = false;
+    int64_t arrived = 0, retired = 0;
+    double next_s = 0;
+  };
+  const int n = d->n_streams, wps = d->windows_per_stream;
+  std::vector<Live> live(n);
+  std::mt19937_64 rng(d->seed);
+  std::vector<hfx_window_done> done(4096);
+  std::vector<int> by_id(d->server.max_streams, -1);
+  r->windows_retired = 0;
+  r->max_concurrent = 0;
+  r->admissions_deferred = 0;
+  r->ingest_deferred = 0;
+  int active = 0, finished = 0, next_admit = 0;
+  const auto t0 = std::chrono::steady_clock::now();
+  auto now_s = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+  double last_retire = 0.0;
+
+  while (finished < n) {
+    const double now = now_s();
+    if (now > d->time_limit_s) return fail(HFX_E_HIP, "workload exceeded its time limit");
+    // admissions, in arrival order
+    while (next_admit < n && d->arrival_s[next_admit] <= now) {
+      Live &L = live[next_admit];
+      const hfx_status rc = hfx_server_admit(srv, &d->streams[next_admit], &L.id);
+      if (rc == HFX_NOT_READY) {
+        ++r->admissions_deferred;
+        break;
+      }
+      SRV_TRY(rc);
+      L.admitted = true;
+      L.next_s = now_s();
+      by_id[L.id] = next_admit;
+      if (r->admit_s) r->admit_s[next_admit] = L.next_s;
+      ++active;
+      r->max_concurrent = std::max(r->max_concurrent, active);
+      ++next_admit;
+    }
+    // window arrivals: the camera delivers the window's new frames
+    for (int i = 0; i < next_admit; ++i) {
+      Live &L = live[i];
+      if (!L.admitted || L.released) continue;
+      while (L.arrived < wps && L.next_s <= now_s()) {
+        const hfx_plan_desc &p = d->streams[i].plan;
+        const hfx_status rc = hfx_server_ingest(srv, L.id, L.arrived == 0 ? p.window : p.stride);
+        if (rc == HFX_NOT_READY) {
+          ++r->ingest_deferred;
+          break;
+        }
+        SRV_TRY(rc);
+        ++L.arrived;
+        std::exponential_distribution<double> ex(d->window_rate_hz[i]);
+        L.next_s += ex(rng);
+      }
+    }
+    int32_t launched = 0;
+    SRV_TRY(hfx_server_schedule(srv, &launched));
+    int32_t k = 0;
+    SRV_TRY(hfx_server_poll(srv, done.data(), (int32_t)done.size(), &k));
+    for (int j = 0; j < k; ++j) {
+      const hfx_window_done &w = done[j];
+      const int i = by_id[w.stream_id];
+      Live &L = live[i];
+      if (r->latency_us) r->latency_us[(int64_t)i * wps + w.window] = w.latency_us;
+      if (r->gpu_us) r->gpu_us[(int64_t)i * wps + w.window] = w.gpu_us;
+      if (i == d->capture_stream) {
+        const float *src = nullptr;
+        SRV_TRY(hfx_server_window_output(srv, w.stream_id, w.window, &src));
+        Stream *st = stream_of(srv, w.stream_id);
+        const size_t elems = (size_t)st->g.out_elems;
+        SRV_HIP(hipMemcpy(d->capture_out + (size_t)w.window * elems, src, elems * 4, hipMemcpyDeviceToHost));
+      }
+      ++L.retired;
+      ++r->windows_retired;
+      last_retire = now_s();
+      if (L.retired == wps) {
+        SRV_TRY(hfx_server_release(srv, L.id));
+        by_id[L.id] = -1;
+        L.released = true;
+        if (r->finish_s) r->finish_s[i] = now_s();
+        --active;
+        ++finished;
+      }
+    }
+  }
+  r->wall_s = last_retire - d->arrival_s[0];
+  return HFX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,132 @@
+/*
+ * hfx_serve.h - serving loop for concurrent acquisition streams on one GPU.
+ *
+ * No counterpart in the reference's hot path: the reference runs one acquisition per
+ * process (Manager::start_pipeline, manager.cc), feeding one BatchQueue ring
+ * (batch_queue.cc:92-174) per graph.  Here a server holds many streams at once: each admitted
+ * stream owns a device-resident frame ring, a moment plan (hfx.h) with its propagation
+ * tables, and an output slot per window position; its windows are replayed from hipGraphs
+ * captured once per window position, on one of several HIP streams ("lanes") so that the
+ * windows of different streams overlap on the device.
+ *
+ * Flow: admit (by free HBM and a free slot) -> ingest frames (synthetic camera into the
+ * ring; refused while it would overwrite frames of windows not yet retired) -> schedule
+ * (launch every ready window of every stream) -> poll (retire finished windows in order,
+ * with their frame-to-moment latency) -> release.
+ *
+ * Latency of a window = time from the ingest event recorded after its last frame was
+ * written to the completion event of its moments (HIP events, same device clock).
+ *
+ * One host thread drives a server; calls on one server must not overlap.
+ */
+#ifndef HFX_SERVE_H_
+#define HFX_SERVE_H_
+
+#include <stdint.h>
+
+#include "hfx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef HFX_NOT_READY
+#define HFX_NOT_READY 5 /* retry later: no slot / memory (admit), ring full (ingest) */
+#endif
+
+typedef struct hfx_server hfx_server;
+
+typedef struct hfx_server_desc {
+  int64_t hbm_budget_bytes;  /* device bytes streams may hold; 0 = free HBM minus reserve */
+  int64_t hbm_reserve_bytes; /* device bytes always left free */
+  int32_t max_streams;       /* concurrent stream slots */
+  int32_t n_lanes;           /* HIP streams for window work (default priority) */
+  int32_t use_graphs;        /* 1: replay window positions from captured hipGraphs */
+} hfx_server_desc;
+
+typedef struct hfx_stream_desc {
+  hfx_plan_desc plan;        /* the stream's moment plan (frame size, T, S, band, optics) */
+  int32_t queue_windows;     /* ready-but-unretired windows the ring holds (ring = T + q*S frames) */
+  hfx_synth_desc camera;     /* synthetic camera feeding the ring */
+} hfx_stream_desc;
+
+typedef struct hfx_window_done {
+  int32_t stream_id;
+  int32_t n_outputs;         /* images per window (3 or 5) */
+  int64_t window;            /* window index within the stream (0, 1, ...) */
+  float latency_us;          /* last frame ingested -> moments complete */
+  float gpu_us;              /* launch -> complete on its lane */
+} hfx_window_done;
+
+typedef struct hfx_server_info {
+  int32_t active_streams;
+  int32_t max_streams;
+  int64_t bytes_held;        /* device bytes held by admitted streams */
+  int64_t bytes_free;        /* hipMemGetInfo free bytes */
+  int64_t windows_launched;
+  int64_t windows_retired;
+} hfx_server_info;
+
+void hfx_server_desc_init(hfx_server_desc *d);
+void hfx_stream_desc_init(hfx_stream_desc *d, int32_t height, int32_t width);
+/* Device bytes a stream of this description holds once admitted (no device work). */
+hfx_status hfx_server_stream_bytes(const hfx_stream_desc *d, int64_t *bytes);
+
+hfx_status hfx_server_create(const hfx_server_desc *d, hfx_server **out);
+/* Waits for all work, frees every stream. */
+hfx_status hfx_server_destroy(hfx_server *s);
+/* HFX_NOT_READY when no slot is free or the stream does not fit the memory budget now. */
+hfx_status hfx_server_admit(hfx_server *s, const hfx_stream_desc *d, int32_t *stream_id);
+/* The camera writes the next n_frames frames of the stream into its ring; HFX_NOT_READY
+ * (nothing written) while that would overwrite frames of a window not yet retired. */
+hfx_status hfx_server_ingest(hfx_server *s, int32_t stream_id, int64_t n_frames);
+/* Launches every window whose frames are in (all streams); *launched = count. */
+hfx_status hfx_server_schedule(hfx_server *s, int32_t *launched);
+/* Retires finished windows of all streams, oldest first per stream: up to cap records. */
+hfx_status hfx_server_poll(hfx_server *s, hfx_window_done *out, int32_t cap, int32_t *n);
+/* Device pointer to the moments of a retired window ([n_out][OH][OW] float); valid until
+ * that stream ingests the frames of window + queue_windows + T/S. */
+hfx_status hfx_server_window_output(hfx_server *s, int32_t stream_id, int64_t window, const float **out);
+/* Waits for the stream's work and frees it (slot and memory return to admission). */
+hfx_status hfx_server_release(hfx_server *s, int32_t stream_id);
+hfx_status hfx_server_info_get(hfx_server *s, hfx_server_info *info);
+
+/* ------------------------------------------------------------------------------------
+ * Native workload driver: Poisson arrivals of streams and of their windows.
+ * Stream i arrives at arrival_s[i] (seconds from start), waits until admitted, then its
+ * windows arrive as a Poisson process of rate window_rate_hz[i] (the first window needs
+ * T frames, every later one S new frames); after windows_per_stream windows are retired
+ * the stream is released.  The loop admits, ingests, schedules and polls continuously.
+ * ---------------------------------------------------------------------------------- */
+typedef struct hfx_workload_desc {
+  hfx_server_desc server;
+  int32_t n_streams;
+  const hfx_stream_desc *streams;  /* n_streams */
+  const double *arrival_s;         /* n_streams, non-decreasing */
+  const double *window_rate_hz;    /* n_streams */
+  int32_t windows_per_stream;
+  uint64_t seed;                   /* window inter-arrival draws */
+  int32_t capture_stream;          /* index whose window outputs are copied to capture_out, or -1 */
+  float *capture_out;              /* host, windows_per_stream * n_out * OH * OW floats */
+  double time_limit_s;             /* the loop stops (HFX_E_HIP) if exceeded */
+} hfx_workload_desc;
+
+typedef struct hfx_workload_result {
+  double wall_s;                   /* first arrival -> last retirement */
+  int64_t windows_retired;
+  int32_t max_concurrent;          /* most streams admitted at once */
+  int32_t admissions_deferred;     /* admit attempts answered HFX_NOT_READY */
+  int64_t ingest_deferred;         /* ingest attempts answered HFX_NOT_READY */
+  float *latency_us;               /* caller array n_streams * windows_per_stream (or NULL) */
+  float *gpu_us;                   /* same shape (or NULL) */
+  double *admit_s;                 /* caller array n_streams (or NULL): admission times */
+  double *finish_s;                /* caller array n_streams (or NULL): release times */
+} hfx_workload_result;
+
+hfx_status hfx_serve_workload(const hfx_workload_desc *d, hfx_workload_result *r);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HFX_SERVE_H_ */
