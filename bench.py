@@ -70,6 +70,51 @@ def time_batch(plan, ring, n_windows, out, steps, warmup, torch):
     return time.perf_counter() - t0
 
 
+def cfg4_leg(args, np, torch):
+    """Large single stream: 2048x2048 u16, disk 900 px, f_s 10 kHz, ASM z 0.35 m, T=128,
+    S=32 over an 8192-frame stream (253 windows), bins [13, 65) with the band split at 52
+    (M0 low 1-4 kHz, M0 high 4-5 kHz): FP32 and FP16-split throughput, errors vs FP64."""
+    from holoflow_amd import hfx
+    from oracle import oracle as O
+
+    H4 = W4 = 2048
+    p = O.config_params("cfg4_2048_asm")
+    d = hfx.synth_desc(H4, W4, disk_radius_px=900.0, fs_hz=10e3, seed=4096)
+    n_frames = args.cfg4_frames
+    ring = torch.empty((n_frames, H4, W4), dtype=torch.int16, device="cuda")
+    phase = torch.zeros(H4 * W4, dtype=torch.float64, device="cuda")
+    for f0 in range(0, n_frames, 128):
+        n = min(128, n_frames - f0)
+        hfx.synth_frames(d, f0, n, out=ring[f0:f0 + n], phase_state=phase)
+    torch.cuda.synchronize()
+    n_win = (n_frames - p.window) // p.stride + 1
+    ref = O.doppler_moments(ring[:p.window].cpu().numpy().view(np.uint16), p, "f64")
+    res = {"workload": "2048x2048 u16, 8192-frame stream, ASM z=0.35 m, T=128 stride 32, bins [13,65) "
+                       "split at 52 (M0, M1, M2, M0 low, M0 high)", "windows_per_step": n_win}
+    out = None
+    for prec in ("fp32", "fp16_split"):
+        plan = hfx.MomentPlan(hfx.MomentConfig(H4, W4, p.window, p.stride, p.bin_begin, p.bin_end, "asm",
+                                               p.lambda_m, p.dx_m, p.dy_m, p.z_m, fs_hz=10e3,
+                                               weight_fs_hz=p.weight_fs_hz, precision=prec,
+                                               split_bin=p.split_bin))
+        if out is None:
+            out = plan.new_output(n_win)
+        dt = time_batch(plan, ring, n_win, out, 2, 1, torch)
+        m = plan.moments(ring, 0).cpu().numpy()
+        res[prec] = {"value": round(n_win * 2 / dt, 2), "unit": "moment frames/s",
+                     "rel_l2_vs_fp64": {k: float(f"{O.rel_l2(m[i], ref[i]):.3e}") for i, k in
+                                        enumerate(("M0", "M1", "M2", "M0_low", "M0_high"))}}
+        plan.close()
+        log(f"cfg4 {prec}: {res[prec]}")
+    res["fp16_split"]["speedup_vs_fp32"] = round(res["fp16_split"]["value"] / res["fp32"]["value"], 4)
+    npix = H4 * W4
+    res["algorithmic_bytes_per_window"] = p.window * npix * 2 + 4 * 52 * npix * 8 + 5 * npix * 4
+    res["path_gbs_fp32"] = round(res["algorithmic_bytes_per_window"] * res["fp32"]["value"] / 1e9, 1)
+    del ring, out
+    torch.cuda.empty_cache()
+    return res
+
+
 def serving_leg(args, np, torch, load=0.7, slots=16):
     """Mixed-stream load: sizes {512^2, 768^2, 1024^2, 2048x512}, T in {32, 64, 128}, S = T/4,
     z ~ U(0.2, 0.6) m, f_s ~ U(20, 75) kHz, ASM.  Each stream's windows arrive as a Poisson
@@ -108,6 +153,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-cfg4", action="store_true")
+    ap.add_argument("--cfg4-frames", type=int, default=8192)
     ap.add_argument("--serving-streams", type=int, default=24)
     ap.add_argument("--serving-windows", type=int, default=128)
     ap.add_argument("--kernel-loop", type=int, default=200,
@@ -232,10 +279,15 @@ def main():
             mp.close()
         mixed["fp32_rel_l2_vs_fp64"] = {k: float(f"{fp32_err[i]:.3e}") for i, k in enumerate(("M0", "M1", "M2"))}
 
+    del out
+    # ---- large single stream (2048^2, T=128, band split)
+    big = None
+    if not args.no_cfg4:
+        big = cfg4_leg(args, np, torch)
+
     # ---- serving: Poisson arrivals of mixed acquisition streams, 16 slots, native loop
     serve = None
     if not args.no_serving:
-        del out
         serve = serving_leg(args, np, torch)
         log(f"serving: {serve}")
 
@@ -291,6 +343,7 @@ def main():
                        "window_gpu_p99": round(float(np.percentile(lat, 99)), 4),
                        "window_period": round(window_period_ms, 4)},
         "mixed": mixed,
+        "cfg4": big,
         "serving": serve,
         "cpu_baseline": cpu,
     }

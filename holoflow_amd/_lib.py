@@ -11,7 +11,8 @@ import os
 import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libhfx.so")
+# HFX_LIBRARY selects another build of the same library (same ABI), e.g. for A/B timing.
+LIB_PATH = os.environ.get("HFX_LIBRARY") or os.path.join(_HERE, "libhfx.so")
 
 HFX_OK = 0
 HFX_E_INVALID = 1
@@ -37,6 +38,7 @@ FLAG_BOTH_SIDEBANDS = 4
 DTYPE_U16 = 1
 DTYPE_F32 = 2
 DTYPE_CF32 = 3
+DTYPE_U8 = 4
 
 REDUCE_POWER2 = 1
 REDUCE_ABS = 2
@@ -133,6 +135,9 @@ SIGNATURES = {
                                _P(C.c_int64), _VP, _VP]),
     "hfx_fftshift": (C.c_int, [_VP, _VP, C.c_int32, C.c_int32, _P(C.c_int64), _P(C.c_int32), _VP]),
     "hfx_convert_u16": (C.c_int, [_VP, _VP, C.c_int64, C.c_int32, _VP]),
+    "hfx_convert_scratch_bytes": (C.c_int64, [C.c_int64]),
+    "hfx_convert": (C.c_int, [_VP, C.c_int32, _VP, C.c_int32, C.c_int32, C.c_int64, _VP, _VP]),
+    "hfx_slide_avg_step": (C.c_int, [_VP, _VP, _VP, C.c_int64, C.c_int32, _VP]),
     "hfx_synth_desc_init": (None, [_P(SynthDesc), C.c_int32, C.c_int32]),
     "hfx_synth_frames": (C.c_int, [_P(SynthDesc), _VP, C.c_int64, C.c_int64, _VP, _VP]),
     # include/hfx_nodes.h
@@ -142,9 +147,9 @@ SIGNATURES = {
     "hfx_node_update": (C.c_int, [_VP, C.c_char_p, C.c_char_p, _VP, _P(C.c_int32)]),
     "hfx_node_execute": (C.c_int, [_VP, _P(_VP), C.c_int32, _P(_VP), C.c_int32]),
     "hfx_node_acquire_input": (C.c_int, [_VP, _P(_VP)]),
-    "hfx_node_push": (C.c_int, [_VP]),
-    "hfx_node_pop": (C.c_int, [_VP, _P(_VP)]),
-    "hfx_node_release_output": (C.c_int, [_VP]),
+    "hfx_node_push": (C.c_int, [_VP, _P(_VP), C.c_int32]),
+    "hfx_node_pop": (C.c_int, [_VP, _P(_VP), C.c_int32]),
+    "hfx_node_release_output": (C.c_int, [_VP, C.c_int32]),
     "hfx_node_destroy": (C.c_int, [_VP]),
 }
 

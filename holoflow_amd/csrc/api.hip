@@ -692,6 +692,33 @@ hfx_status hfx_convert_u16(const uint16_t *in, void *out, int64_t n, int32_t to_
   return HFX_OK;
 }
 
+int64_t hfx_convert_scratch_bytes(int64_t n) { return (int64_t)(hfx::minmax_blocks(n) + 1) * 8; }
+
+hfx_status hfx_convert(const void *in, int32_t in_dtype, void *out, int32_t out_dtype, int32_t strategy,
+                       int64_t n, void *scratch, hfx_stream_t stream) {
+  if ((!in || !out) && n > 0) return fail(HFX_E_INVALID, "null argument");
+  if (n < 0) return fail(HFX_E_INVALID, "negative element count");
+  const bool ok =
+      (strategy == HFX_CONV_REAL && (in_dtype == HFX_DTYPE_U8 || in_dtype == HFX_DTYPE_U16 || in_dtype == HFX_DTYPE_F32) &&
+       (out_dtype == HFX_DTYPE_F32 || out_dtype == HFX_DTYPE_CF32) && !(in_dtype == HFX_DTYPE_F32 && out_dtype == HFX_DTYPE_F32)) ||
+      (strategy == HFX_CONV_SCALED && in_dtype == HFX_DTYPE_F32 && (out_dtype == HFX_DTYPE_U8 || out_dtype == HFX_DTYPE_U16)) ||
+      ((strategy == HFX_CONV_MODULUS || strategy == HFX_CONV_ARGUMENT) && in_dtype == HFX_DTYPE_CF32 && out_dtype == HFX_DTYPE_F32);
+  if (!ok) return fail(HFX_E_INVALID, "unsupported conversion configuration");
+  if (strategy == HFX_CONV_SCALED && !scratch && n > 0) return fail(HFX_E_INVALID, "Scaled needs a scratch buffer");
+  if (hfx::launch_convert(in, in_dtype, out, out_dtype, strategy, n, scratch, S(stream)) != 0)
+    return fail(HFX_E_HIP, "conversion launch failed");
+  return HFX_OK;
+}
+
+hfx_status hfx_slide_avg_step(const float *frame, float *old_slot, float *running, int64_t n, int32_t window,
+                              hfx_stream_t stream) {
+  if ((!frame || !old_slot || !running) && n > 0) return fail(HFX_E_INVALID, "null argument");
+  if (window <= 0 || n < 0) return fail(HFX_E_INVALID, "window must be positive");
+  if (hfx::launch_slide_avg(frame, old_slot, running, n, window, S(stream)) != 0)
+    return fail(HFX_E_HIP, "sliding average launch failed");
+  return HFX_OK;
+}
+
 void hfx_synth_desc_init(hfx_synth_desc *d, int32_t height, int32_t width) {
   std::memset(d, 0, sizeof(*d));
   d->height = height;

@@ -228,6 +228,157 @@ __global__ __launch_bounds__(row_threads<W>()) void k_dft_row(DftArgs a) {
   }
 }
 
+// ----------------------------------------------------------------------- pipelined front
+// Same result as k_dft_row for a single LDS group of planes, with the waves split by role:
+// waves 0-3 (producers) run the MFMA time DFT of the block's next row into registers while
+// waves 4-7 (consumers) run the row FFTs of the current row's planes out of LDS and store
+// them; then the producers scatter the next row into LDS.  HBM loads and MFMAs of one row
+// overlap the LDS/VALU work and the stores of the previous one.
+// Producers own 128 pixels each (two 64-pixel MFMA tiles); eight consumer waves share the
+// planes.  FP32 only: its operand loads are one 4-frame step each, so a short prefetch
+// (PRE steps across rows, batches of KB steps inside a row) fits the register budget of
+// 3-4 waves per SIMD.
+#ifndef FRONT_CW
+#define FRONT_CW 4
+#endif
+template <int W> constexpr int front_pw() { return W / 128; }
+template <int W> constexpr int front_cw() { return FRONT_CW; }
+template <int W> constexpr int front_threads() { return (front_pw<W>() + front_cw<W>()) * 64; }
+
+template <int W, int NBT, int MODE>
+__global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
+  static_assert(MODE == 0, "the pipelined front kernel runs the FP32 time DFT");
+  using F = WaveFft<W>;
+  constexpr int TPW = 2;
+  constexpr int NPW = front_pw<W>();
+  constexpr int NT = front_threads<W>();
+  constexpr int PRE = 2, KB = 2;
+  constexpr int E = F::E;
+  constexpr int GB = row_group<W>();
+  __shared__ float2 scr[GB * F::SCR];  // one padded row per plane
+  __shared__ float2 twl[F::TW];
+  extern __shared__ float4 wmat[];     // operand matrix of the MFMA loop
+
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const bool producer = wave < NPW;    // wave-uniform role
+  const int rw = producer ? wave : wave - NPW;
+  dft_stage_matrix<NBT, MODE>(wmat, a.wm_hi, a.wm_lo, a.r.T, tid, NT);
+  F::build_twiddles(twl, a.tw, tid, NT);
+  __syncthreads();
+
+  const long long plane_elems = (long long)a.H * W;
+  const bool lens = a.lx != nullptr;
+  const int G = gridDim.x;
+  const int y0 = xcd_tile(blockIdx.x, G);
+  const int nr = y0 < a.H ? (a.H - 1 - y0) / G + 1 : 0;
+
+  f4 acc[TPW][NBT][4];
+  uint2 pre[TPW][PRE];
+
+  // producers: the row's band spectra for their 128 pixels
+  auto dft_row = [&](int y, int lane) {
+    dft_tile<NBT, MODE, TPW, PRE, KB>(a.r, wmat, (long long)y * W + rw * 64 * TPW, lane, pre, acc);
+  };
+  auto prefetch_row = [&](int y, int lane) {
+    dft_prefetch<MODE, TPW, PRE>(a.r, (long long)y * W + rw * 64 * TPW, lane, pre);
+  };
+  // producers: MFMA registers -> padded LDS rows (lens applied for Fresnel)
+  auto scatter = [&](int y, int lane) {
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) {
+      const int x0 = (rw * TPW + i) * 64 + 4 * (lane & 15);
+      float2 L[4];
+      if (lens) {
+        const float2 ly = a.ly[y];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) L[t] = cmul(a.lx[x0 + t], ly);
+      }
+#pragma unroll
+      for (int bt = 0; bt < NBT; ++bt) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int bin = bt * 8 + 2 * (lane >> 4) + h;
+          if (bin >= a.nbc) continue;
+          float2 v[4];
+          dft_bin_values(a, acc[i][bt], h, bin, v);
+#pragma unroll
+          for (int side = 0; side < 2; ++side) {
+            int q;
+            if (side == 0) {
+              if (bin >= a.nb) continue;
+              q = bin;
+            } else {
+              if (!a.both) continue;
+              const int j = a.nb - bin;
+              if (j < 0 || j >= a.nb) continue;
+              q = a.nb + j;
+            }
+            float2 *dst = scr + q * F::SCR;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+              float2 c = side ? make_float2(v[t].x, -v[t].y) : v[t];
+              dst[F::pad(x0 + t)] = lens ? cmul(c, L[t]) : c;
+            }
+          }
+        }
+      }
+    }
+  };
+  // consumers: one wave per plane row: forward row FFT, then the column-blocked store
+  auto transform = [&](int y, int lane) {
+    for (int q = rw; q < a.n_planes; q += front_cw<W>()) {
+      float2 *ws = scr + q * F::SCR;
+      float2 v[E];
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = ws[F::pad(lane + 64 * m)];
+      F::run(v, ws, twl, lane, false);
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int x = lane + 64 * m;
+        a.planes[(long long)q * plane_elems + (long long)(x / a.blk_c) * a.H * a.blk_c +
+                 (long long)y * a.blk_c + (x % a.blk_c)] = v[m];
+      }
+    }
+  };
+
+  // The two roles run separate loops (wave-uniform branch) that meet at the same number of
+  // workgroup barriers: 1 + 2 per row.  Keeping them apart keeps the producers' MFMA
+  // accumulators out of the consumers' register allocation.
+  if (producer) {
+    const int lane0 = tid & 63;
+    if (nr > 0) {
+      prefetch_row(y0, lane0);
+      dft_row(y0, lane0);
+      if (nr > 1) prefetch_row(y0 + G, lane0);
+      scatter(y0, lane0);
+    }
+    __syncthreads();
+    for (int i = 0; i < nr; ++i) {
+      const int y = y0 + i * G;
+      int lane;
+      asm volatile("v_mov_b32 %0, %1" : "=v"(lane) : "v"(tid & 63));
+      if (i + 1 < nr) {
+        dft_row(y + G, lane);
+        if (i + 2 < nr) prefetch_row(y + 2 * G, lane);
+      }
+      __syncthreads();  // consumers are done with row y's planes
+      if (i + 1 < nr) scatter(y + G, lane);
+      __syncthreads();  // row y + G's planes are in LDS
+    }
+  } else {
+    __syncthreads();
+    for (int i = 0; i < nr; ++i) {
+      // opaque per-row lane index: keeps loop-invariant LDS addresses out of registers
+      int lane;
+      asm volatile("v_mov_b32 %0, %1" : "=v"(lane) : "v"(tid & 63));
+      transform(y0 + i * G, lane);
+      __syncthreads();
+      __syncthreads();
+    }
+  }
+}
+
 // ----------------------------------------------------------------------- launch
 static DftArgs make_args(const DftLaunch &L) {
   DftArgs a{};
@@ -297,7 +448,9 @@ template <int W, int NBT, int MODE> static int launch_row(const DftArgs &a, hipS
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   const unsigned grid = (unsigned)(a.H < cus ? a.H : cus);
-  if (a.n_planes <= row_group<W>())
+  if (MODE == 0 && a.n_planes <= row_group<W>())
+    hipLaunchKernelGGL((k_front<W, NBT, 0>), dim3(grid), dim3(front_threads<W>()), lds, st, a);
+  else if (a.n_planes <= row_group<W>())
     hipLaunchKernelGGL((k_dft_row<W, NBT, MODE, false>), dim3(grid), dim3(row_threads<W>()), lds, st, a);
   else
     hipLaunchKernelGGL((k_dft_row<W, NBT, MODE, true>), dim3(grid), dim3(row_threads<W>()), lds, st, a);

@@ -82,14 +82,16 @@ __device__ __forceinline__ long long ring_frame(const DftRing &r, long long n) {
 
 // The wave owns TPW adjacent 64-pixel tiles starting at p0w (a multiple of 4); tile i
 // covers p0w + 64*i + {4*j + t}.  npix % 4 == 0.
-template <int MODE, int TPW>
+// PRE (FP32 only) lowers the number of prefetched steps where registers are scarce.
+template <int MODE, int TPW, int PRE = dft_pre_steps<MODE>()>
 __device__ __forceinline__ void dft_prefetch(const DftRing &r, long long p0w, int lane,
-                                             uint2 (&pre)[TPW][dft_pre_steps<MODE>()]) {
+                                             uint2 (&pre)[TPW][PRE]) {
+  static_assert(MODE == 0 || PRE == 8, "16-bit modes prefetch one 32-frame step");
   const int sub = lane >> 4;
   const long long p0 = p0w + 4 * (lane & 15);
   if (MODE == 0) {
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
+    for (int s = 0; s < PRE; ++s) {
       if (4 * s < r.T) {
         const long long f = ring_frame(r, 4 * s + sub);
 #pragma unroll
@@ -115,9 +117,9 @@ __device__ __forceinline__ void unpack4(uint2 raw, float (&x)[4]) {
 
 // MFMA loop over the window, starting from the prefetched first 32 frames `pre` and
 // loading the rest one batch ahead of its use.
-template <int NBT, int MODE, int TPW>
+template <int NBT, int MODE, int TPW, int PRE = dft_pre_steps<MODE>(), int KB = 8 / TPW>
 __device__ __forceinline__ void dft_tile(const DftRing &r, const void *lds, long long p0w, int lane,
-                                         const uint2 (&pre)[TPW][dft_pre_steps<MODE>()],
+                                         const uint2 (&pre)[TPW][PRE],
                                          f4 (&acc)[TPW][NBT][4]) {
   const int T = r.T;
   const int sub = lane >> 4;
@@ -148,8 +150,8 @@ __device__ __forceinline__ void dft_tile(const DftRing &r, const void *lds, long
       }
     };
     const int steps = T / 4;
-    // steps beyond the prefetched 8 come in batches of K, one batch ahead
-    constexpr int K = 8 / TPW;
+    // steps beyond the prefetched PRE come in batches of K, one batch ahead
+    constexpr int K = KB;
     uint2 nxt[K][TPW];
     auto load_batch = [&](int s0) {
 #pragma unroll
@@ -161,9 +163,9 @@ __device__ __forceinline__ void dft_tile(const DftRing &r, const void *lds, long
         }
       }
     };
-    if (steps > 8) load_batch(8);
+    if (steps > PRE) load_batch(PRE);
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
+    for (int s = 0; s < PRE; ++s) {
       if (s < steps) {
         uint2 raw[TPW];
 #pragma unroll
@@ -171,7 +173,7 @@ __device__ __forceinline__ void dft_tile(const DftRing &r, const void *lds, long
         step(raw, s);
       }
     }
-    for (int s0 = 8; s0 < steps; s0 += K) {
+    for (int s0 = PRE; s0 < steps; s0 += K) {
       uint2 cur[K][TPW];
 #pragma unroll
       for (int k = 0; k < K; ++k)

@@ -99,6 +99,10 @@ int launch_strided_copy(const void *in, void *out, int elem_size, int ndim, cons
 int launch_multiply(const void *a, int a_complex, const long long *sa, const void *b, int b_complex,
                     const long long *sb, int ndim, const long long *shape, void *out, hipStream_t st);
 int launch_convert_u16(const uint16_t *in, void *out, long long n, int to_complex, hipStream_t st);
+int minmax_blocks(long long n);
+int launch_convert(const void *in, int in_dt, void *out, int out_dt, int strategy, long long n,
+                   void *scratch, hipStream_t st);
+int launch_slide_avg(const float *x, float *old_slot, float *ravg, long long n, int ws, hipStream_t st);
 int launch_dft_axis(const void *in, int dtype, long long outer, int n, long long inner, int kb,
                     int ke, float scale, float2 *out, hipStream_t st);
 

@@ -79,14 +79,55 @@ void require_device(const TDesc &d, const char *who) {
 }
 
 // ================================================================== Conversion
+int hfx_code(DType d) {
+  switch (d) {
+    case DType::U8: return HFX_DTYPE_U8;
+    case DType::U16: return HFX_DTYPE_U16;
+    case DType::F32: return HFX_DTYPE_F32;
+    case DType::CF32: return HFX_DTYPE_CF32;
+  }
+  return 0;
+}
+
+// (input dtype, target, strategy) -> output dtype: the reference's table (conversion.cu:
+// 138-150) plus U16 -> F32 Real, which the path's graph uses (graph_builder.cc:242).
+bool conversion_out(DType in, DType target, int strategy, DType &out) {
+  struct Row {
+    DType in, target;
+    int strategy;
+  };
+  static const Row rows[] = {
+      {DType::U8, DType::F32, HFX_CONV_REAL},      {DType::U8, DType::CF32, HFX_CONV_REAL},
+      {DType::U16, DType::F32, HFX_CONV_REAL},     {DType::U16, DType::CF32, HFX_CONV_REAL},
+      {DType::F32, DType::U8, HFX_CONV_SCALED},    {DType::F32, DType::U16, HFX_CONV_SCALED},
+      {DType::F32, DType::CF32, HFX_CONV_REAL},    {DType::CF32, DType::F32, HFX_CONV_MODULUS},
+      {DType::CF32, DType::F32, HFX_CONV_ARGUMENT},
+  };
+  for (const auto &r : rows)
+    if (r.in == in && r.target == target && r.strategy == strategy) {
+      out = target;
+      return true;
+    }
+  return false;
+}
+
+int strategy_code(const std::string &s) {
+  if (s == "Real") return HFX_CONV_REAL;
+  if (s == "Scaled") return HFX_CONV_SCALED;
+  if (s == "Modulus") return HFX_CONV_MODULUS;
+  if (s == "Argument") return HFX_CONV_ARGUMENT;
+  throw std::invalid_argument("Unknown strategy type for conversion: " + s);
+}
+
 class ConversionTask : public ISyncTask {
  public:
   hipStream_t st;
   size_t n;
-  int to_complex;
+  int in_dt, out_dt, strategy;
+  curaii::unique_device_ptr<std::byte> scratch;
   OpResult execute(SyncCtx &c) override {
-    HFX_CHECK(hfx_convert_u16(reinterpret_cast<const uint16_t *>(c.inputs[0].data()), c.outputs[0].data(),
-                              (int64_t)n, to_complex, hs(st)));
+    HFX_CHECK(hfx_convert(c.inputs[0].data(), in_dt, c.outputs[0].data(), out_dt, strategy, (int64_t)n,
+                          scratch.get(), hs(st)));
     return OpResult::Ok;
   }
 };
@@ -95,26 +136,41 @@ class ConversionFactory : public ISyncTaskFactory {
  public:
   InferResult infer(std::span<const TDesc> in, const Json &s) const override {
     const char *who = "ConversionFactory";
-    check(in.size() == 1, who, "expected exactly one input");
-    require_device(in[0], who);
-    check(in[0].is_contiguous(), who, "input must be contiguous");
-    const DType target = dtype_from_string(s.at("target").str());
-    const std::string strategy = s.at("strategy").str();
-    // The path converts camera U16 frames; the reference lacks U16 -> F32 (conversion.cu:
-    // 138-150) although the graph asks for it (graph_builder.cc:242).  Display conversions
-    // (Scaled / Modulus / Argument to U8) are after the moments and not part of this build.
-    check(in[0].dtype == DType::U16, who, "only U16 input is supported");
-    check(strategy == "Real", who, "only the Real strategy is supported");
-    check(target == DType::F32 || target == DType::CF32, who, "target must be F32 or CF32");
-    return sync_result({in[0]}, {TDesc(in[0].shape, target, in[0].mem_loc)});
+    const std::string ts = s.at("target").str();
+    if (ts != "U8" && ts != "U16" && ts != "F32" && ts != "CF32")
+      throw std::invalid_argument("Unknown target type for conversion: " + ts);
+    const DType target = dtype_from_string(ts);
+    const int strategy = strategy_code(s.at("strategy").str());
+    check(in.size() == 1, who, "Expected exactly one input tensor");
+    check(in[0].mem_loc == MemLoc::Device, who, "Input tensor must be in device memory");
+    check(in[0].is_contiguous(), who, "Input tensor must be C-contiguous");
+    DType out;
+    check(conversion_out(in[0].dtype, target, strategy, out), who, "Unsupported conversion configuration");
+    return sync_result({in[0]}, {TDesc(in[0].shape, out, MemLoc::Device)});
   }
   std::unique_ptr<ISyncTask> create(std::span<const TDesc> in, const Json &s, const SyncCreateCtx &ctx) const override {
     const auto r = infer(in, s);
     auto t = std::make_unique<ConversionTask>();
     t->st = ctx.stream;
     t->n = in[0].num_elements();
-    t->to_complex = r.output_descs[0].dtype == DType::CF32;
+    t->in_dt = hfx_code(in[0].dtype);
+    t->out_dt = hfx_code(r.output_descs[0].dtype);
+    t->strategy = strategy_code(s.at("strategy").str());
+    if (t->strategy == HFX_CONV_SCALED)
+      t->scratch = curaii::make_unique_device_ptr<std::byte>((size_t)hfx_convert_scratch_bytes((int64_t)t->n));
     return t;
+  }
+  std::unique_ptr<ISyncTask> update(std::unique_ptr<ISyncTask> old, std::span<const TDesc> in, const Json &s,
+                                    const SyncCreateCtx &ctx) const override {
+    (void)infer(in, s);
+    auto *t = dynamic_cast<ConversionTask *>(old.get());
+    if (t && t->n == in[0].num_elements() && t->in_dt == hfx_code(in[0].dtype) &&
+        t->strategy == strategy_code(s.at("strategy").str()) &&
+        t->out_dt == hfx_code(dtype_from_string(s.at("target").str()))) {
+      t->st = ctx.stream;
+      return old;
+    }
+    return create(in, s, ctx);
   }
 };
 
@@ -1149,10 +1205,247 @@ class BatchQueueFactory : public IAsyncTaskFactory {
   }
 };
 
+// ================================================================== SlidingAverage (async)
+// Rolling mean of the last window_size pushed frames (slide_avg.cu:52-313): pushes land in
+// a ring of target_capacity + window_size slots; each push folds the new frame in and the
+// frame window_size pushes older out of a running sum, and stores the running mean in that
+// older frame's slot, which the consumer pops in order.  An optional host U8 input of shape
+// [1] discards the push when zero; `discard_first` drops the first accepted pushes.
+class SlidingAverage : public IAsyncTask {
+ public:
+  TDesc idesc, odesc;
+  size_t ws = 1, cap = 1, discard_first = 0, discarded = 0, nb_slots = 0, element_size = 0;
+  hipStream_t producer = nullptr;
+  curaii::unique_device_ptr<std::byte> buf;
+  curaii::unique_device_ptr<float> ravg;
+  std::atomic<size_t> avg_idx{0}, write_idx{0}, read_idx{0};
+  Json settings;
+
+  size_t dist(size_t r, size_t w) const { return w >= r ? w - r : w + nb_slots - r; }
+  std::optional<TView> acquire_input(int index) override {
+    if (index != 0) throw std::out_of_range("SlidingAverage::acquire_input: invalid index");
+    if (nb_slots - dist(read_idx.load(std::memory_order_acquire), write_idx.load()) <= 1) return std::nullopt;
+    auto &st = storage_access().owned_input_storage(0);
+    st.ptr = buf.get() + write_idx.load(std::memory_order_relaxed) * element_size;
+    return TView{idesc, &st};
+  }
+  OpResult try_push(AsyncPushCtx &ctx) override {
+    const bool invalid = ctx.inputs.size() == 2 && *reinterpret_cast<const uint8_t *>(ctx.inputs[1].data()) == 0;
+    if (invalid || discarded < discard_first) {
+      if (!invalid) ++discarded;
+      storage_access().owned_input_storage(0).ptr = nullptr;
+      return OpResult::Ok;
+    }
+    const size_t w = write_idx.load(std::memory_order_relaxed), a = avg_idx.load(std::memory_order_relaxed);
+    auto *frame = reinterpret_cast<const float *>(buf.get() + w * element_size);
+    auto *old = reinterpret_cast<float *>(buf.get() + a * element_size);
+    HFX_CHECK(hfx_slide_avg_step(frame, old, ravg.get(), (int64_t)(element_size / 4), (int32_t)ws, hs(producer)));
+    HIP_CHECK(hipStreamSynchronize(producer));
+    avg_idx.store(a + 1 == nb_slots ? 0 : a + 1, std::memory_order_release);
+    storage_access().owned_input_storage(0).ptr = nullptr;
+    write_idx.store(w + 1 == nb_slots ? 0 : w + 1, std::memory_order_release);
+    return OpResult::Ok;
+  }
+  OpResult try_pop(AsyncPopCtx &ctx) override {
+    const size_t r = read_idx.load(std::memory_order_relaxed);
+    if (dist(r, write_idx.load(std::memory_order_acquire)) <= ws) return OpResult::NotReady;
+    auto &st = storage_access().owned_output_storage(0);
+    st.ptr = buf.get() + r * element_size;
+    ctx.outputs[0] = TView{odesc, &st};
+    return OpResult::Ok;
+  }
+  void release_output(int index) override {
+    if (index != 0) throw std::out_of_range("SlidingAverage::release_output: invalid index");
+    const size_t r = read_idx.load(std::memory_order_relaxed);
+    storage_access().owned_output_storage(0).ptr = nullptr;
+    read_idx.store(r + 1 == nb_slots ? 0 : r + 1, std::memory_order_release);
+  }
+};
+
+class SlidingAverageFactory : public IAsyncTaskFactory {
+ public:
+  InferResult infer(std::span<const TDesc> in, const Json &s) const override {
+    auto check_ = [](bool c, const char *what) {
+      if (!c) throw std::logic_error(what);
+    };
+    const long long cap = s.at("target_capacity").integer(), ws = s.at("window_size").integer();
+    check_(in.size() == 1 || in.size() == 2, "expected one input and optional valid");
+    const TDesc &d = in[0];
+    check_(d.dtype == DType::F32, "input must be float32");
+    check_(d.rank() == 3, "input must be 3D");
+    check_(d.shape[0] == 1, "input batch size must be 1");
+    check_(d.mem_loc == MemLoc::Device, "input must be in device memory");
+    check_(d.is_contiguous(), "input must be contiguous");
+    check_(cap > 0, "target_capacity must be greater or equal than zero");
+    check_(ws > 0, "window_size must be greater or equal than zero");
+    if (in.size() == 2) {
+      check_(in[1].dtype == DType::U8, "valid input must be uint8");
+      check_(in[1].mem_loc == MemLoc::Host, "valid input must be in host memory");
+      check_(in[1].shape == std::vector<size_t>{1}, "valid input must have shape [1]");
+    }
+    InferResult r;
+    r.input_descs.assign(in.begin(), in.end());
+    r.output_descs = {d};
+    r.owned_inputs = in.size() == 1 ? std::vector<bool>{true} : std::vector<bool>{true, false};
+    r.owned_outputs = {true};
+    r.kind = TaskKind::Async;
+    return r;
+  }
+  std::unique_ptr<IAsyncTask> create(std::span<const TDesc> in, const Json &s, const AsyncCreateCtx &ctx) const override {
+    const auto r = infer(in, s);
+    auto t = std::make_unique<SlidingAverage>();
+    t->settings = s;
+    t->idesc = in[0];
+    t->odesc = r.output_descs[0];
+    t->cap = (size_t)s.at("target_capacity").integer();
+    t->ws = (size_t)s.at("window_size").integer();
+    t->discard_first = (size_t)s.get_int("discard_first", 0);
+    t->nb_slots = t->cap + t->ws;
+    t->element_size = in[0].num_bytes();
+    t->producer = ctx.producer_stream;
+    t->buf = curaii::make_unique_device_ptr<std::byte>(t->nb_slots * t->element_size);
+    t->ravg = curaii::make_unique_device_ptr<float>(t->element_size / 4);
+    HIP_CHECK(hipMemsetAsync(t->ravg.get(), 0, t->element_size, ctx.producer_stream));
+    HIP_CHECK(hipMemsetAsync(t->buf.get(), 0, t->nb_slots * t->element_size, ctx.producer_stream));
+    HIP_CHECK(hipStreamSynchronize(ctx.producer_stream));
+    t->avg_idx = t->nb_slots - t->ws;
+    t->write_idx = 0;
+    t->read_idx = t->nb_slots - 1;
+    return t;
+  }
+  std::unique_ptr<IAsyncTask> update(std::unique_ptr<IAsyncTask> old, std::span<const TDesc> in, const Json &s,
+                                     const AsyncCreateCtx &ctx) const override {
+    (void)infer(in, s);
+    auto *t = dynamic_cast<SlidingAverage *>(old.get());
+    if (t && t->settings == s && t->idesc == in[0]) {
+      t->producer = ctx.producer_stream;
+      return old;
+    }
+    return create(in, s, ctx);
+  }
+};
+
+// ================================================================== DualReaderBatchQueue (async)
+// Ring with two read cursors (dual_reader_batch_queue.cc:48-181): each pop exposes the current
+// frame I[n], the delayed frame I[n - h] with h = (window_size - 1) / 2 (zero scratch until
+// the delayed reader has a frame) and a host U8 flag that turns 1 once n >= window_size - 1.
+// A slot is reused only after both readers released it.
+class DualReaderBatchQueue : public IAsyncTask {
+ public:
+  TDesc idesc, odesc;
+  size_t ws = 1, delay = 0, slot_count = 0, input_size = 1, element_size = 0;
+  curaii::unique_device_ptr<std::byte> dbuf, dscratch;
+  curaii::unique_host_ptr<std::byte> hbuf, hscratch;
+  std::byte *buf = nullptr, *scratch = nullptr;
+  std::atomic<size_t> write_idx{0}, delayed_idx{0};
+  size_t current_idx = 0, sequence = 0;
+  bool pop_active = false, delayed_active = false, cur_released = false, del_released = false;
+
+  size_t inc(size_t i) const { return i + 1 == slot_count ? 0 : i + 1; }
+  size_t dist(size_t b, size_t e) const { return e >= b ? e - b : e + slot_count - b; }
+  std::optional<TView> acquire_input(int index) override {
+    if (index != 0) throw std::out_of_range("DualReaderBatchQueue::acquire_input: invalid index");
+    if (slot_count - dist(delayed_idx.load(std::memory_order_acquire), write_idx.load()) <= input_size)
+      return std::nullopt;
+    auto &st = storage_access().owned_input_storage(0);
+    st.ptr = buf + write_idx.load(std::memory_order_relaxed) * element_size;
+    return TView{idesc, &st};
+  }
+  OpResult try_push(AsyncPushCtx &) override {
+    size_t w = write_idx.load(std::memory_order_relaxed) + input_size;
+    if (w >= slot_count) w = 0;
+    storage_access().owned_input_storage(0).ptr = nullptr;
+    write_idx.store(w, std::memory_order_release);
+    return OpResult::Ok;
+  }
+  OpResult try_pop(AsyncPopCtx &ctx) override {
+    if (dist(current_idx, write_idx.load(std::memory_order_acquire)) < 1) return OpResult::NotReady;
+    if (pop_active) throw std::logic_error("DualReaderBatchQueue::try_pop: prior output was not released");
+    auto &cur = storage_access().owned_output_storage(0);
+    auto &del = storage_access().owned_output_storage(1);
+    cur.ptr = buf + current_idx * element_size;
+    delayed_active = sequence >= delay;
+    del.ptr = delayed_active ? buf + delayed_idx.load(std::memory_order_relaxed) * element_size : scratch;
+    ctx.outputs[0] = TView{odesc, &cur};
+    ctx.outputs[1] = TView{odesc, &del};
+    *reinterpret_cast<uint8_t *>(ctx.outputs[2].data()) = sequence >= ws - 1 ? 1 : 0;
+    cur_released = del_released = false;
+    pop_active = true;
+    return OpResult::Ok;
+  }
+  void release_output(int index) override {
+    if (index != 0 && index != 1) throw std::out_of_range("DualReaderBatchQueue::release_output: invalid index");
+    if (!pop_active) throw std::logic_error("DualReaderBatchQueue::release_output: no active output");
+    storage_access().owned_output_storage((size_t)index).ptr = nullptr;
+    (index == 0 ? cur_released : del_released) = true;
+    if (!cur_released || !del_released) return;
+    current_idx = inc(current_idx);
+    if (delayed_active) delayed_idx.store(inc(delayed_idx.load(std::memory_order_relaxed)), std::memory_order_release);
+    ++sequence;
+    pop_active = false;
+  }
+};
+
+class DualReaderBatchQueueFactory : public IAsyncTaskFactory {
+ public:
+  InferResult infer(std::span<const TDesc> in, const Json &s) const override {
+    const char *who = "DualReaderBatchQueueFactory";
+    const long long cap = s.at("target_capacity").integer(), ws = s.at("window_size").integer();
+    check(in.size() == 1, who, "task must have exactly one input");
+    const TDesc &d = in[0];
+    check(d.rank() > 0, who, "input rank must be positive");
+    check(d.shape[0] > 0, who, "input leading dimension must be positive");
+    check(d.is_contiguous(), who, "input must be contiguous");
+    check(cap > 0, who, "target_capacity must be positive");
+    check(ws > 0, who, "window_size must be positive");
+    TDesc out = d;
+    out.shape[0] = 1;
+    InferResult r;
+    r.input_descs = {d};
+    r.output_descs = {out, out, TDesc({1}, DType::U8, MemLoc::Host)};
+    r.owned_inputs = {true};
+    r.owned_outputs = {true, true, false};
+    r.kind = TaskKind::Async;
+    return r;
+  }
+  std::unique_ptr<IAsyncTask> create(std::span<const TDesc> in, const Json &s, const AsyncCreateCtx &ctx) const override {
+    const auto r = infer(in, s);
+    auto q = std::make_unique<DualReaderBatchQueue>();
+    q->idesc = in[0];
+    q->odesc = r.output_descs[0];
+    q->ws = (size_t)s.at("window_size").integer();
+    q->delay = (q->ws - 1) / 2;
+    q->input_size = in[0].shape[0];
+    q->element_size = in[0].num_bytes() / q->input_size;
+    // multiple of the input batch holding capacity + one batch + the delay + one slot
+    // (dual_reader_batch_queue.cc:242-246)
+    const size_t minimum = (size_t)s.at("target_capacity").integer() + q->input_size + q->delay + 1;
+    q->slot_count = ((minimum + q->input_size - 1) / q->input_size) * q->input_size;
+    const size_t bytes = q->slot_count * q->element_size;
+    if (in[0].mem_loc == MemLoc::Host) {
+      q->hbuf = curaii::make_unique_host_ptr<std::byte>(bytes);
+      q->hscratch = curaii::make_unique_host_ptr<std::byte>(q->element_size);
+      q->buf = q->hbuf.get();
+      q->scratch = q->hscratch.get();
+      std::memset(q->scratch, 0, q->element_size);
+    } else {
+      q->dbuf = curaii::make_unique_device_ptr<std::byte>(bytes);
+      q->dscratch = curaii::make_unique_device_ptr<std::byte>(q->element_size);
+      q->buf = q->dbuf.get();
+      q->scratch = q->dscratch.get();
+      HIP_CHECK(hipMemsetAsync(q->scratch, 0, q->element_size, ctx.consumer_stream));
+      HIP_CHECK(hipStreamSynchronize(ctx.consumer_stream));
+    }
+    return q;
+  }
+};
+
 }  // namespace
 
 void register_hip_nodes(Registry &r) {
   r.register_async("BatchQueue", std::make_unique<BatchQueueFactory>());
+  r.register_async("SlidingAverage", std::make_unique<SlidingAverageFactory>());
+  r.register_async("DualReaderBatchQueue", std::make_unique<DualReaderBatchQueueFactory>());
   r.register_sync("Conversion", std::make_unique<ConversionFactory>());
   r.register_sync("Reshape", std::make_unique<ReshapeFactory>());
   r.register_sync("Slice", std::make_unique<SliceFactory>());
@@ -1184,9 +1477,15 @@ struct hfx_node : public IOStorageAccess {
   hipStream_t stream = nullptr;
   std::unique_ptr<ISyncTask> sync;
   std::unique_ptr<IAsyncTask> async;
-  Storage owned_in{MemLoc::Device, 0, nullptr}, owned_out{MemLoc::Device, 0, nullptr};
-  Storage &owned_input_storage(size_t) override { return owned_in; }
-  Storage &owned_output_storage(size_t) override { return owned_out; }
+  Storage owned_in[4] = {}, owned_out[4] = {};
+  Storage &owned_input_storage(size_t i) override {
+    if (i >= 4) throw std::out_of_range("owned input index");
+    return owned_in[i];
+  }
+  Storage &owned_output_storage(size_t i) override {
+    if (i >= 4) throw std::out_of_range("owned output index");
+    return owned_out[i];
+  }
 };
 
 namespace {
@@ -1212,7 +1511,7 @@ template <typename F> hfx_status guarded(F &&f) {
     f();
     hfx::set_last_error("");
     return HFX_OK;
-  } catch (const std::invalid_argument &e) {
+  } catch (const std::logic_error &e) {  // invalid_argument, out_of_range, logic_error
     hfx::set_last_error(e.what());
     return HFX_E_INVALID;
   } catch (const curaii::HfxError &e) {
@@ -1364,35 +1663,56 @@ hfx_status hfx_node_acquire_input(hfx_node *n, void **slot) {
   return g != HFX_OK ? g : rc;
 }
 
-hfx_status hfx_node_push(hfx_node *n) {
-  if (!n || !n->async) return HFX_E_INVALID;
+hfx_status hfx_node_push(hfx_node *n, void *const *extra_inputs, int32_t n_extra) {
+  if (!n || !n->async || n_extra < 0 || (n_extra > 0 && !extra_inputs)) return HFX_E_INVALID;
   return guarded([&] {
+    if ((size_t)n_extra + 1 != n->in_descs.size())
+      throw std::invalid_argument("push: expected " + std::to_string(n->in_descs.size() - 1) + " extra inputs");
+    std::vector<Storage> st(n->in_descs.size());
+    std::vector<TView> views(n->in_descs.size());
+    views[0] = TView{n->in_descs[0], &n->owned_in[0]};
+    for (int i = 0; i < n_extra; ++i) {
+      const TDesc &d = n->in_descs[i + 1];
+      st[i + 1] = Storage{d.mem_loc, d.num_bytes(), static_cast<std::byte *>(extra_inputs[i])};
+      views[i + 1] = TView{d, &st[i + 1]};
+    }
     std::atomic<bool> cancelled{false};
-    AsyncPushCtx ctx{{}, &cancelled};
+    AsyncPushCtx ctx{std::span<TView>(views), &cancelled};
     (void)n->async->try_push(ctx);
   });
 }
 
-hfx_status hfx_node_pop(hfx_node *n, void **data) {
-  if (!n || !n->async || !data) return HFX_E_INVALID;
+hfx_status hfx_node_pop(hfx_node *n, void **outputs, int32_t n_outputs) {
+  if (!n || !n->async || !outputs) return HFX_E_INVALID;
   hfx_status rc = HFX_OK;
   hfx_status g = guarded([&] {
+    const auto &od = n->info.output_descs;
+    if ((size_t)n_outputs != od.size())
+      throw std::invalid_argument("pop: expected " + std::to_string(od.size()) + " outputs");
+    std::vector<Storage> st(od.size());
+    std::vector<TView> views(od.size());
+    for (size_t i = 0; i < od.size(); ++i) {
+      // outputs the task does not own are caller storage the task writes into
+      if (!n->info.owned_outputs[i]) {
+        st[i] = Storage{od[i].mem_loc, od[i].num_bytes(), static_cast<std::byte *>(outputs[i])};
+        views[i] = TView{od[i], &st[i]};
+      }
+    }
     std::atomic<bool> cancelled{false};
-    TView out;
-    AsyncPopCtx ctx{std::span<TView>(&out, 1), &cancelled};
+    AsyncPopCtx ctx{std::span<TView>(views), &cancelled};
     if (n->async->try_pop(ctx) != OpResult::Ok) {
       rc = HFX_NOT_READY;
-      *data = nullptr;
-    } else {
-      *data = out.data();
+      return;
     }
+    for (size_t i = 0; i < od.size(); ++i)
+      if (n->info.owned_outputs[i]) outputs[i] = views[i].data();
   });
   return g != HFX_OK ? g : rc;
 }
 
-hfx_status hfx_node_release_output(hfx_node *n) {
+hfx_status hfx_node_release_output(hfx_node *n, int32_t index) {
   if (!n || !n->async) return HFX_E_INVALID;
-  return guarded([&] { n->async->release_output(0); });
+  return guarded([&] { n->async->release_output(index); });
 }
 
 hfx_status hfx_node_destroy(hfx_node *n) {

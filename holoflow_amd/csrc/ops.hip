@@ -125,6 +125,126 @@ int launch_convert_u16(const uint16_t *in, void *out, long long n, int to_comple
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// ---------------------------------------------------------------- Conversion (all pairs)
+// dtype codes: 1 U16, 2 F32, 3 CF32, 4 U8; strategy: 0 Real, 1 Scaled, 2 Modulus,
+// 3 Argument (conversion.cu:138-150, kernels :156-215).
+__device__ __forceinline__ float conv_load_real(const void *in, int dt, long long i) {
+  switch (dt) {
+    case 1: return (float)reinterpret_cast<const uint16_t *>(in)[i];
+    case 4: return (float)reinterpret_cast<const uint8_t *>(in)[i];
+    default: return reinterpret_cast<const float *>(in)[i];
+  }
+}
+
+__global__ void k_convert(const void *in, int in_dt, void *out, int out_dt, int strategy, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (in_dt == 3) {  // CF32 -> F32: modulus (hypot) or argument (atan2)
+    const float2 z = reinterpret_cast<const float2 *>(in)[i];
+    reinterpret_cast<float *>(out)[i] = strategy == 3 ? atan2f(z.y, z.x) : hypotf(z.x, z.y);
+    return;
+  }
+  const float v = conv_load_real(in, in_dt, i);
+  if (out_dt == 3)
+    reinterpret_cast<float2 *>(out)[i] = make_float2(v, 0.f);
+  else
+    reinterpret_cast<float *>(out)[i] = v;
+}
+
+// min / max of n floats: per-block partials, then one block folds them (order-free ops)
+__global__ __launch_bounds__(256) void k_minmax_partial(const float *in, long long n, float2 *partial) {
+  float lo = INFINITY, hi = -INFINITY;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const float v = in[i];
+    lo = fminf(lo, v);
+    hi = fmaxf(hi, v);
+  }
+  __shared__ float2 red[256];
+  red[threadIdx.x] = make_float2(lo, hi);
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      red[threadIdx.x].x = fminf(red[threadIdx.x].x, red[threadIdx.x + s].x);
+      red[threadIdx.x].y = fmaxf(red[threadIdx.x].y, red[threadIdx.x + s].y);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_minmax_final(float2 *partial, int nblocks) {
+  float lo = INFINITY, hi = -INFINITY;
+  for (int i = threadIdx.x; i < nblocks; i += 256) {
+    lo = fminf(lo, partial[i].x);
+    hi = fmaxf(hi, partial[i].y);
+  }
+  __shared__ float2 red[256];
+  red[threadIdx.x] = make_float2(lo, hi);
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      red[threadIdx.x].x = fminf(red[threadIdx.x].x, red[threadIdx.x + s].x);
+      red[threadIdx.x].y = fmaxf(red[threadIdx.x].y, red[threadIdx.x + s].y);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[nblocks] = red[0];
+}
+
+// F32 -> U8/U16 scaled to the tensor's [min, max] (conversion.cu:177-195)
+__global__ void k_scaled(const float *in, void *out, int out_dt, long long n, const float2 *mm) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float lo = mm->x, hi = mm->y;
+  const float top = out_dt == 4 ? 255.f : 65535.f;
+  const float scale = top / (hi - lo);
+  const int val = (int)((in[i] - lo) * scale + 0.5f);
+  const float c = fminf(fmaxf((float)val, 0.f), top);
+  if (out_dt == 4)
+    reinterpret_cast<uint8_t *>(out)[i] = (uint8_t)c;
+  else
+    reinterpret_cast<uint16_t *>(out)[i] = (uint16_t)c;
+}
+
+int minmax_blocks(long long n) { return (int)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024); }
+
+int launch_convert(const void *in, int in_dt, void *out, int out_dt, int strategy, long long n,
+                   void *scratch, hipStream_t st) {
+  if (n == 0) return 0;
+  const unsigned grid = (unsigned)((n + 255) / 256);
+  if (strategy == 1) {
+    const int nb = minmax_blocks(n);
+    float2 *part = reinterpret_cast<float2 *>(scratch);
+    hipLaunchKernelGGL(k_minmax_partial, dim3(nb), dim3(256), 0, st, reinterpret_cast<const float *>(in), n, part);
+    hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(256), 0, st, part, nb);
+    hipLaunchKernelGGL(k_scaled, dim3(grid), dim3(256), 0, st, reinterpret_cast<const float *>(in), out, out_dt,
+                       n, part + nb);
+  } else {
+    hipLaunchKernelGGL(k_convert, dim3(grid), dim3(256), 0, st, in, in_dt, out, out_dt, strategy, n);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ---------------------------------------------------------------- SlidingAverage step
+// ravg = (ravg + x / ws) - old / ws, then the old slot takes ravg (slide_avg.cu:102-125,
+// 200-215: two kernels and a copy; each operation rounds to FP32 in the same order here).
+__global__ void k_slide_avg(const float *x, float *old_slot, float *ravg, long long n, float ws) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float r = ravg[i];
+  r = r + x[i] / ws;
+  r = r - old_slot[i] / ws;
+  ravg[i] = r;
+  old_slot[i] = r;
+}
+
+int launch_slide_avg(const float *x, float *old_slot, float *ravg, long long n, int ws, hipStream_t st) {
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_slide_avg, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, old_slot, ravg, n,
+                     (float)ws);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 // Gather of a strided view (holonp::Copy, copy.cu:40,117-125; the pieces of
 // holonp::Concatenate).  Strides and offset in bytes, as TDesc has them.  Element t of the
 // view goes to out[(t / out_block) * out_pitch + t % out_block]: one input of a
@@ -156,10 +276,66 @@ __global__ void k_strided_copy(const uint8_t *in, uint8_t *out, CopyGeo g) {
     for (int b = 0; b < g.esz; ++b) out[o * g.esz + b] = in[src + b];
 }
 
+// Contiguous fast path: 16 bytes per lane, 4 vectors in flight per lane, grid sized to the
+// device (8 blocks of 256 per CU), grid-stride over the buffer.
+__global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ in, uint4 *__restrict__ out,
+                                                long long n16) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n16; i += 4 * stride) {
+    const uint4 a = in[i], b = in[i + stride], c = in[i + 2 * stride], d = in[i + 3 * stride];
+    out[i] = a;
+    out[i + stride] = b;
+    out[i + 2 * stride] = c;
+    out[i + 3 * stride] = d;
+  }
+  for (; i < n16; i += stride) out[i] = in[i];
+}
+
+static int launch_copy_bytes(const uint8_t *in, uint8_t *out, long long bytes, hipStream_t st) {
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const long long n16 = bytes / 16;
+  if (n16 > 0) {
+    long long blocks = (n16 + 255) / 256;
+    if (blocks > 8LL * cus) blocks = 8LL * cus;
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, st,
+                       reinterpret_cast<const uint4 *>(in), reinterpret_cast<uint4 *>(out), n16);
+  }
+  const long long tail = bytes - n16 * 16;
+  if (tail > 0) {
+    CopyGeo g{};
+    g.ndim = 1;
+    g.esz = 1;
+    g.shape[0] = tail;
+    g.in_stride[0] = 1;
+    g.in_offset = n16 * 16;
+    g.total = tail;
+    g.out_block = tail;
+    g.out_pitch = tail;
+    hipLaunchKernelGGL(k_strided_copy, dim3(1), dim3(256), 0, st, in, out + n16 * 16, g);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int launch_strided_copy(const void *in, void *out, int elem_size, int ndim, const long long *shape,
                         const long long *in_strides, long long in_offset, long long out_block,
                         long long out_pitch, hipStream_t st) {
   if (ndim < 1 || ndim > 8) return -1;
+  {
+    // a C-contiguous view written contiguously is a plain byte copy
+    bool contiguous = out_block <= 0;
+    long long expect = elem_size, total = 1;
+    for (int k = ndim - 1; k >= 0; --k) {
+      if (shape[k] != 1 && in_strides[k] != expect) contiguous = false;
+      expect *= shape[k];
+      total *= shape[k];
+    }
+    const uint8_t *src = reinterpret_cast<const uint8_t *>(in) + in_offset;
+    if (contiguous && ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(out)) % 16) == 0)
+      return total == 0 ? 0 : launch_copy_bytes(src, reinterpret_cast<uint8_t *>(out), total * elem_size, st);
+  }
   CopyGeo g;
   g.ndim = ndim;
   g.esz = elem_size;

@@ -12,6 +12,7 @@ import ctypes as C
 import json
 from typing import Dict, List, Optional, Sequence
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -140,10 +141,12 @@ def _copy_raw(src_ptr: int, d: Dict, dst_ptr: int, stream) -> None:
 
 
 class AsyncNode(Node):
-    """An asynchronous node (BatchQueue): push input batches, pop output batches."""
+    """An asynchronous node (BatchQueue, SlidingAverage, DualReaderBatchQueue): push input
+    batches into owned slots, pop outputs (copied out, then released)."""
 
-    def push(self, t: torch.Tensor) -> bool:
-        """Copies t into the next owned input slot and pushes it; False when full."""
+    def push(self, t: torch.Tensor, *extra: int) -> bool:
+        """Copies t into the next owned input slot and pushes it; False when full.
+        `extra` are the further inputs (host U8 scalars), as 0/1 ints."""
         slot = C.c_void_p()
         rc = lib().hfx_node_acquire_input(self._h, C.byref(slot))
         if rc == _lib.HFX_NOT_READY:
@@ -151,18 +154,39 @@ class AsyncNode(Node):
         check(rc)
         t = t.contiguous()
         _copy_raw(t.data_ptr(), desc(t.shape, _NAMES[t.dtype]), slot.value, self._stream)
-        check(lib().hfx_node_push(self._h))
+        flags = [(C.c_uint8 * 1)(int(v)) for v in extra]
+        arr = (C.c_void_p * max(1, len(flags)))(*[C.cast(f, C.c_void_p).value for f in flags])
+        check(lib().hfx_node_push(self._h, arr, len(flags)))
         return True
 
-    def pop(self) -> Optional[torch.Tensor]:
-        """Copies the next output batch out and releases it; None when not ready."""
-        data = C.c_void_p()
-        rc = lib().hfx_node_pop(self._h, C.byref(data))
+    def pop(self):
+        """None when not ready; else the outputs: owned ones copied to new device tensors,
+        host U8 flags as ints."""
+        ods = self.output_descs
+        owned = self.info["owned_outputs"]
+        flags = {i: (C.c_uint8 * 1)(0) for i, o in enumerate(owned) if not o}
+        ptrs = (C.c_void_p * len(ods))(*[C.cast(flags[i], C.c_void_p).value if i in flags else None
+                                          for i in range(len(ods))])
+        rc = lib().hfx_node_pop(self._h, ptrs, len(ods))
         if rc == _lib.HFX_NOT_READY:
             return None
         check(rc)
-        od = self.output_descs[0]
-        out = torch.empty(od["shape"], dtype=DTYPES[od["dtype"]], device="cuda")
-        _copy_raw(data.value, desc(od["shape"], od["dtype"]), out.data_ptr(), self._stream)
-        check(lib().hfx_node_release_output(self._h))
-        return out
+        outs = []
+        for i, od in enumerate(ods):
+            if i in flags:
+                outs.append(int(flags[i][0]))
+                continue
+            if od["mem_loc"] == "Host":       # a host-memory queue: plain host copy
+                n = int(np.prod(od["shape"])) * _ESZ[od["dtype"]]
+                host = np.empty(n, np.uint8)
+                C.memmove(host.ctypes.data, ptrs[i], n)
+                outs.append(torch.from_numpy(host).view(DTYPES[od["dtype"]]).reshape(od["shape"]))
+                continue
+            out = torch.empty(od["shape"], dtype=DTYPES[od["dtype"]], device="cuda")
+            _copy_raw(ptrs[i], desc(od["shape"], od["dtype"]), out.data_ptr(), self._stream)
+            outs.append(out)
+        torch.cuda.synchronize()
+        for i, o in enumerate(owned):
+            if o:
+                check(lib().hfx_node_release_output(self._h, i))
+        return outs[0] if len(outs) == 1 else outs

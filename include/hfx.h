@@ -192,6 +192,22 @@ hfx_status hfx_fftshift(const void *in, void *out, int32_t elem_size, int32_t nd
 hfx_status hfx_convert_u16(const uint16_t *in, void *out, int64_t n, int32_t to_complex,
                            hfx_stream_t stream);
 
+/* holotask::syncs::Conversion, every pair the reference supports (conversion.cu:138-150,
+ * 156-215, 299-340) plus U16 -> F32 Real (which the path's graph asks for,
+ * graph_builder.cc:242).  Scaled maps the tensor's [min, max] to the full U8/U16 range,
+ * int((x - min) * (top / (max - min)) + 0.5) clamped; it needs `scratch` of
+ * hfx_convert_scratch_bytes(n) device bytes (per-block min/max partials). */
+enum hfx_dtype { HFX_DTYPE_U16 = 1, HFX_DTYPE_F32 = 2, HFX_DTYPE_CF32 = 3, HFX_DTYPE_U8 = 4 };
+enum hfx_conv_strategy { HFX_CONV_REAL = 0, HFX_CONV_SCALED = 1, HFX_CONV_MODULUS = 2, HFX_CONV_ARGUMENT = 3 };
+int64_t hfx_convert_scratch_bytes(int64_t n);
+hfx_status hfx_convert(const void *in, int32_t in_dtype, void *out, int32_t out_dtype, int32_t strategy,
+                       int64_t n, void *scratch, hfx_stream_t stream);
+
+/* One push of holotask::asyncs::SlidingAverage (slide_avg.cu:102-125, 200-215):
+ * running = (running + frame / window) - old_slot / window, then old_slot = running. */
+hfx_status hfx_slide_avg_step(const float *frame, float *old_slot, float *running, int64_t n, int32_t window,
+                              hfx_stream_t stream);
+
 /* ------------------------------------------------------------------------------------
  * Seeded synthetic interferograms (replaces the camera / holofile sources for tests and
  * benchmarks; no reference counterpart on the path).  phase_state: device array of

@@ -52,11 +52,16 @@ hfx_status hfx_node_execute(hfx_node *node, void *const *inputs, int32_t n_input
                             int32_t n_outputs);
 
 /* IAsyncTask owned slots and queue operations (tasks.hh:136-151, 208-220):
- * acquire_input -> producer writes the slot -> push; pop -> consumer reads -> release. */
+ * acquire_input -> producer writes the slot -> push; pop -> consumer reads -> release.
+ * push: extra_inputs are the inputs after the owned first one (e.g. SlidingAverage's host
+ * U8 validity flag), as storage base pointers.  pop: outputs[i] receives the storage
+ * pointer of each output the node owns; for an output it does not own (DualReaderBatchQueue's
+ * host U8 validity flag) the caller passes the buffer the node writes.  HFX_NOT_READY when
+ * nothing can be popped.  release_output(index) releases one owned output. */
 hfx_status hfx_node_acquire_input(hfx_node *node, void **slot);
-hfx_status hfx_node_push(hfx_node *node);
-hfx_status hfx_node_pop(hfx_node *node, void **data);
-hfx_status hfx_node_release_output(hfx_node *node);
+hfx_status hfx_node_push(hfx_node *node, void *const *extra_inputs, int32_t n_extra);
+hfx_status hfx_node_pop(hfx_node *node, void **outputs, int32_t n_outputs);
+hfx_status hfx_node_release_output(hfx_node *node, int32_t index);
 
 hfx_status hfx_node_destroy(hfx_node *node);
 

@@ -86,6 +86,25 @@ def test_cfg1_1024_fresnel_one_window(gpu):
     _check(p, ring, 16)
 
 
+def test_cfg4_2048_asm_band_split_one_window(gpu):
+    # 2048^2, T=128, f_s 10 kHz, bins [13, 65) with the band split at 52 (5 outputs):
+    # FP32 and FP16-split time DFT against the FP64 oracle
+    p = O.config_params("cfg4_2048_asm")
+    assert (p.bin_begin, p.bin_end, p.split_bin) == (13, 65, 52)
+    ring = _frames(2048, 2048, 128, fs=10e3, radius=900)
+    from holoflow_amd import hfx
+
+    win = ring.cpu().numpy().view(np.uint16)
+    ref = O.doppler_moments(win, p, "f64")
+    for precision in ("fp32", "fp16_split"):
+        plan = hfx.MomentPlan(_cfg(p, precision))
+        out = plan.moments(ring, 0).cpu().numpy()
+        plan.close()
+        errs = [rel_l2(out[i], ref[i]) for i in range(5)]
+        print("cfg4", precision, errs)
+        assert max(errs) < TOL_FP32, (precision, errs)
+
+
 @pytest.mark.parametrize("precision,tol", [("fp16_split", 1e-5), ("fp16", 1e-3), ("bf16", 2e-2)])
 def test_mixed_precision_time_dft_vs_fp64(gpu, precision, tol):
     p = O.config_params("cfg1_1024_fresnel")

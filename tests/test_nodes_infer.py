@@ -262,3 +262,46 @@ def test_doppler_moments_infer():
         nodes.infer("DopplerMoments", dict(st, prop="bogus"), [D([32, 64, 64], "U16")])
     with pytest.raises(HfxInvalidArgument):
         nodes.infer("DopplerMoments", st, [D([32, 64, 64], "F32")])
+
+
+# ---------------------------------------------------------------- display chain (next to the path)
+@pytest.mark.parametrize("dt,target,strategy,out", [
+    ("U8", "F32", "Real", "F32"), ("U8", "CF32", "Real", "CF32"), ("U16", "CF32", "Real", "CF32"),
+    ("U16", "F32", "Real", "F32"), ("F32", "U8", "Scaled", "U8"), ("F32", "U16", "Scaled", "U16"),
+    ("F32", "CF32", "Real", "CF32"), ("CF32", "F32", "Modulus", "F32"), ("CF32", "F32", "Argument", "F32")])
+def test_conversion_table(dt, target, strategy, out):
+    r = nodes.infer("Conversion", {"target": target, "strategy": strategy}, [D([2, 3], dt)])
+    assert r["output_descs"][0]["dtype"] == out and r["output_descs"][0]["shape"] == [2, 3]
+
+
+@pytest.mark.parametrize("dt,target,strategy", [("F32", "U8", "Real"), ("CF32", "U8", "Scaled"),
+                                                ("U16", "U8", "Scaled"), ("F32", "F32", "Real")])
+def test_conversion_rejects_unsupported(dt, target, strategy):
+    with pytest.raises(HfxInvalidArgument):
+        nodes.infer("Conversion", {"target": target, "strategy": strategy}, [D([4], dt)])
+    with pytest.raises(HfxInvalidArgument):
+        nodes.infer("Conversion", {"target": "X", "strategy": "Real"}, [D([4], "U8")])
+
+
+def test_sliding_average_infer():
+    st = {"target_capacity": 4, "window_size": 3}
+    r = nodes.infer("SlidingAverage", st, [D([1, 8, 8], "F32")])
+    assert r["kind"] == "Async" and r["output_descs"][0]["shape"] == [1, 8, 8]
+    r = nodes.infer("SlidingAverage", st, [D([1, 8, 8], "F32"), D([1], "U8", mem_loc="Host")])
+    assert r["owned_inputs"] == [True, False]
+    for bad in ([D([2, 8, 8], "F32")], [D([1, 8, 8], "U16")], [D([8, 8], "F32")],
+                [D([1, 8, 8], "F32"), D([1], "U8")]):
+        with pytest.raises(HfxInvalidArgument):
+            nodes.infer("SlidingAverage", st, bad)
+    with pytest.raises(HfxInvalidArgument):
+        nodes.infer("SlidingAverage", {"target_capacity": 0, "window_size": 3}, [D([1, 8, 8], "F32")])
+
+
+def test_dual_reader_batch_queue_infer():
+    r = nodes.infer("DualReaderBatchQueue", {"target_capacity": 4, "window_size": 5}, [D([8, 3, 4, 4], "CF32")])
+    o = r["output_descs"]
+    assert [d["shape"] for d in o] == [[1, 3, 4, 4], [1, 3, 4, 4], [1]]
+    assert o[2]["dtype"] == "U8" and o[2]["mem_loc"] == "Host"
+    assert r["owned_outputs"] == [True, True, False]
+    with pytest.raises(HfxInvalidArgument):
+        nodes.infer("DualReaderBatchQueue", {"target_capacity": 0, "window_size": 5}, [D([8, 3], "F32")])
