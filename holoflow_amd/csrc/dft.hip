@@ -201,7 +201,7 @@ __global__ __launch_bounds__(row_threads<W>()) void k_dft_row(DftArgs a) {
         float2 v[E];
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = ws[F::pad(lane + 64 * m)];
-        F::run(v, ws, twl, lane, false);
+        F::template run<false>(v, ws, twl, lane);
 #pragma unroll
         for (int m = 0; m < E; ++m) {
           const int x = lane + 64 * m;
@@ -234,25 +234,24 @@ __global__ __launch_bounds__(row_threads<W>()) void k_dft_row(DftArgs a) {
 // waves 4-7 (consumers) run the row FFTs of the current row's planes out of LDS and store
 // them; then the producers scatter the next row into LDS.  HBM loads and MFMAs of one row
 // overlap the LDS/VALU work and the stores of the previous one.
-// Producers own 128 pixels each (two 64-pixel MFMA tiles); eight consumer waves share the
-// planes.  FP32 only: its operand loads are one 4-frame step each, so a short prefetch
-// (PRE steps across rows, batches of KB steps inside a row) fits the register budget of
-// 3-4 waves per SIMD.
-#ifndef FRONT_CW
-#define FRONT_CW 4
-#endif
+// Producers own 128 pixels each (two 64-pixel MFMA tiles) and stream their frame loads
+// through a K-step register ring that runs on across rows (DftStream); four consumer waves
+// share the planes.  FP32 only, for T in {32, 64, 128}.
 template <int W> constexpr int front_pw() { return W / 128; }
-template <int W> constexpr int front_cw() { return FRONT_CW; }
+template <int W> constexpr int front_cw() { return 4; }
 template <int W> constexpr int front_threads() { return (front_pw<W>() + front_cw<W>()) * 64; }
+#ifndef HFX_FRONT_RING
+#define HFX_FRONT_RING 8
+#endif
+constexpr int front_ring(int T) { return T / 4 < HFX_FRONT_RING ? T / 4 : HFX_FRONT_RING; }
 
-template <int W, int NBT, int MODE>
+template <int W, int NBT, int MODE, int T>
 __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
   static_assert(MODE == 0, "the pipelined front kernel runs the FP32 time DFT");
   using F = WaveFft<W>;
   constexpr int TPW = 2;
   constexpr int NPW = front_pw<W>();
   constexpr int NT = front_threads<W>();
-  constexpr int PRE = 2, KB = 2;
   constexpr int E = F::E;
   constexpr int GB = row_group<W>();
   __shared__ float2 scr[GB * F::SCR];  // one padded row per plane
@@ -274,14 +273,13 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
   const int nr = y0 < a.H ? (a.H - 1 - y0) / G + 1 : 0;
 
   f4 acc[TPW][NBT][4];
-  uint2 pre[TPW][PRE];
+  DftStream<NBT, TPW, T, front_ring(T)> stream;
+  auto px0 = [&](int y) { return (long long)y * W + rw * 64 * TPW; };
 
   // producers: the row's band spectra for their 128 pixels
   auto dft_row = [&](int y, int lane) {
-    dft_tile<NBT, MODE, TPW, PRE, KB>(a.r, wmat, (long long)y * W + rw * 64 * TPW, lane, pre, acc);
-  };
-  auto prefetch_row = [&](int y, int lane) {
-    dft_prefetch<MODE, TPW, PRE>(a.r, (long long)y * W + rw * 64 * TPW, lane, pre);
+    const bool has_next = y + G < a.H;
+    stream.row(a.r, reinterpret_cast<const float *>(wmat), px0(y), px0(has_next ? y + G : y), has_next, lane, acc);
   };
   // producers: MFMA registers -> padded LDS rows (lens applied for Fresnel)
   auto scatter = [&](int y, int lane) {
@@ -332,7 +330,7 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
       float2 v[E];
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = ws[F::pad(lane + 64 * m)];
-      F::run(v, ws, twl, lane, false);
+      F::template run<false>(v, ws, twl, lane);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int x = lane + 64 * m;
@@ -348,9 +346,8 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
   if (producer) {
     const int lane0 = tid & 63;
     if (nr > 0) {
-      prefetch_row(y0, lane0);
+      stream.start(a.r, px0(y0), lane0);
       dft_row(y0, lane0);
-      if (nr > 1) prefetch_row(y0 + G, lane0);
       scatter(y0, lane0);
     }
     __syncthreads();
@@ -358,10 +355,7 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
       const int y = y0 + i * G;
       int lane;
       asm volatile("v_mov_b32 %0, %1" : "=v"(lane) : "v"(tid & 63));
-      if (i + 1 < nr) {
-        dft_row(y + G, lane);
-        if (i + 2 < nr) prefetch_row(y + 2 * G, lane);
-      }
+      if (i + 1 < nr) dft_row(y + G, lane);
       __syncthreads();  // consumers are done with row y's planes
       if (i + 1 < nr) scatter(y + G, lane);
       __syncthreads();  // row y + G's planes are in LDS
@@ -448,9 +442,15 @@ template <int W, int NBT, int MODE> static int launch_row(const DftArgs &a, hipS
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   const unsigned grid = (unsigned)(a.H < cus ? a.H : cus);
-  if (MODE == 0 && a.n_planes <= row_group<W>())
-    hipLaunchKernelGGL((k_front<W, NBT, 0>), dim3(grid), dim3(front_threads<W>()), lds, st, a);
-  else if (a.n_planes <= row_group<W>())
+  const int T = a.r.T;
+  if (MODE == 0 && a.n_planes <= row_group<W>() && (T == 32 || T == 64 || T == 128)) {
+    if (T == 32)
+      hipLaunchKernelGGL((k_front<W, NBT, 0, 32>), dim3(grid), dim3(front_threads<W>()), lds, st, a);
+    else if (T == 64)
+      hipLaunchKernelGGL((k_front<W, NBT, 0, 64>), dim3(grid), dim3(front_threads<W>()), lds, st, a);
+    else
+      hipLaunchKernelGGL((k_front<W, NBT, 0, 128>), dim3(grid), dim3(front_threads<W>()), lds, st, a);
+  } else if (a.n_planes <= row_group<W>())
     hipLaunchKernelGGL((k_dft_row<W, NBT, MODE, false>), dim3(grid), dim3(row_threads<W>()), lds, st, a);
   else
     hipLaunchKernelGGL((k_dft_row<W, NBT, MODE, true>), dim3(grid), dim3(row_threads<W>()), lds, st, a);

@@ -255,4 +255,60 @@ __device__ __forceinline__ void dft_tile(const DftRing &r, const void *lds, long
   }
 }
 
+// FP32 time DFT of a sequence of rows with a continuous load stream: a ring of K steps
+// (4 frames x TPW tiles each) in flight per lane, refilled one step at a time right after
+// the step is unpacked, running on into the first K steps of the next row during the last
+// K steps of the current one.  With STEPS = T/4 a multiple of K, step s of every row lives
+// in slot s % K.  Holds K*TPW*2 registers of loads in flight with no double-buffer copy.
+template <int NBT, int TPW, int T, int K> struct DftStream {
+  static constexpr int STEPS = T / 4;
+  static_assert(STEPS % K == 0, "ring depth must divide the steps of a row");
+  uint2 ring[K][TPW];
+
+  __device__ __forceinline__ void load(const DftRing &r, long long p0w, int lane, int s, int slot) {
+    const long long f = ring_frame(r, 4 * s + (lane >> 4));
+    const long long p0 = p0w + 4 * (lane & 15);
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) ring[slot][i] = dft_load4(r, f, p0 + 64 * i);
+  }
+  // steps 0 .. K-1 of the first row
+  __device__ __forceinline__ void start(const DftRing &r, long long p0w, int lane) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) load(r, p0w, lane, k, k);
+  }
+  // D = Wm * X for the row at p0w (operand matrix lw with pitch T + 4 floats); the ring
+  // refills from this row, then from the row at p0w_next when has_next.
+  __device__ __forceinline__ void row(const DftRing &r, const float *lw, long long p0w, long long p0w_next,
+                                      bool has_next, int lane, f4 (&acc)[TPW][NBT][4]) {
+#pragma unroll
+    for (int i = 0; i < TPW; ++i)
+#pragma unroll
+      for (int bt = 0; bt < NBT; ++bt)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[i][bt][t] = f4{0.f, 0.f, 0.f, 0.f};
+    constexpr int TP = T + 4;
+#pragma unroll
+    for (int s = 0; s < STEPS; ++s) {
+      const int slot = s % K;
+      float x[TPW][4];
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) unpack4(ring[slot][i], x[i]);
+      if (s + K < STEPS)
+        load(r, p0w, lane, s + K, slot);
+      else if (has_next)
+        load(r, p0w_next, lane, s + K - STEPS, slot);
+      const int n = 4 * s + (lane >> 4);
+#pragma unroll
+      for (int bt = 0; bt < NBT; ++bt) {
+        const float w = lw[(bt * 16 + (lane & 15)) * TP + n];
+#pragma unroll
+        for (int i = 0; i < TPW; ++i)
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+            acc[i][bt][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(w, x[i][t], acc[i][bt][t], 0, 0, 0);
+      }
+    }
+  }
+};
+
 }  // namespace hfx

@@ -64,13 +64,60 @@ template <int R> __device__ __forceinline__ float2 wr(int p, float sgn) {
   }
 }
 
-// In-register DFT of length R (natural order in and out), kernel exp(sgn*2*pi*i*k*n/R).
+// t * exp(S*2*pi*i*p/R) for a compile-time sign S (+1 inverse, -1 forward) and a
+// compile-time exponent p after unrolling: quarter turns are exchanges and negations, the
+// rest a complex multiply by the exact constant.
+template <int R, int S> __device__ __forceinline__ float2 twr(float2 t, int p) {
+  p %= R;
+  if (p == 0) return t;
+  if (4 * p == R) return S > 0 ? make_float2(-t.y, t.x) : make_float2(t.y, -t.x);      // * S i
+  if (2 * p == R) return make_float2(-t.x, -t.y);                                       // * -1
+  if (4 * p == 3 * R) return S > 0 ? make_float2(t.y, -t.x) : make_float2(-t.y, t.x);  // * -S i
+  return cmul(t, wr<R>(p, (float)S));
+}
+
+// Radix-2/3/4 butterflies with the kernel sign fixed at compile time.
+template <int R, int S> struct ButterflyS;
+template <int S> struct ButterflyS<2, S> {
+  __device__ __forceinline__ static void run(float2 *v) {
+    const float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+template <int S> struct ButterflyS<4, S> {
+  __device__ __forceinline__ static void run(float2 *v) {
+    const float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+    const float2 s13 = cadd(v[1], v[3]), d13 = twr<4, S>(csub(v[1], v[3]), 1);
+    v[0] = cadd(s02, s13);
+    v[2] = csub(s02, s13);
+    v[1] = cadd(d02, d13);
+    v[3] = csub(d02, d13);
+  }
+};
+template <int S> struct ButterflyS<3, S> {
+  __device__ __forceinline__ static void run(float2 *v) {
+    // w = exp(S*2*pi*i/3) = -1/2 + S*i*sqrt(3)/2
+    const float c = -0.5f;
+    const float s = (float)S * 0.86602540378443864676f;
+    const float2 a = v[0], b = v[1], d = v[2];
+    const float2 t = cadd(b, d);
+    const float2 u = csub(b, d);
+    const float2 m = make_float2(a.x + c * t.x, a.y + c * t.y);
+    const float2 q = make_float2(-s * u.y, s * u.x);  // s * i * u
+    v[0] = cadd(a, t);
+    v[1] = cadd(m, q);
+    v[2] = csub(m, q);
+  }
+};
+
+// In-register DFT of length R (natural order in and out), kernel exp(S*2*pi*i*k*n/R).
 // Composite lengths split as R = A*B (A = 4, 2 or 3):
 //   Y[n2][k1] = W_R^{n2 k1} * DFT_A(x[B n1 + n2])[k1],  X[k1 + A k2] = DFT_B(Y[.][k1])[k2].
-template <int R> struct RegDft {
-  __device__ __forceinline__ static void run(float2 (&v)[R], float sgn) {
+template <int R, int S> struct RegDft {
+  __device__ __forceinline__ static void run(float2 (&v)[R]) {
     if constexpr (R == 2 || R == 3 || R == 4) {
-      Butterfly<R>::run(v, sgn);
+      ButterflyS<R, S>::run(v);
     } else {
       constexpr int A = (R % 4 == 0) ? 4 : (R % 2 == 0) ? 2 : 3;
       constexpr int B = R / A;
@@ -80,16 +127,16 @@ template <int R> struct RegDft {
         float2 t[A];
 #pragma unroll
         for (int n1 = 0; n1 < A; ++n1) t[n1] = v[B * n1 + n2];
-        Butterfly<A>::run(t, sgn);
+        ButterflyS<A, S>::run(t);
 #pragma unroll
-        for (int k1 = 0; k1 < A; ++k1) y[n2][k1] = (n2 * k1 == 0) ? t[k1] : cmul(t[k1], wr<R>(n2 * k1, sgn));
+        for (int k1 = 0; k1 < A; ++k1) y[n2][k1] = twr<R, S>(t[k1], n2 * k1);
       }
 #pragma unroll
       for (int k1 = 0; k1 < A; ++k1) {
         float2 u[B];
 #pragma unroll
         for (int n2 = 0; n2 < B; ++n2) u[n2] = y[n2][k1];
-        RegDft<B>::run(u, sgn);
+        RegDft<B, S>::run(u);
 #pragma unroll
         for (int k2 = 0; k2 < B; ++k2) v[k1 + A * k2] = u[k2];
       }
@@ -140,9 +187,8 @@ template <int N> struct WaveFft {
     }
   }
 
-  template <int NS>
-  __device__ __forceinline__ static void pass(float2 (&v)[E], float2 *scr, const float2 *tw, int lane,
-                                              float sgn) {
+  template <int NS, int S>
+  __device__ __forceinline__ static void pass(float2 (&v)[E], float2 *scr, const float2 *tw, int lane) {
     constexpr int R = wave_pass_radix(N / NS, E);
     static_assert(R >= 2, "bad factorisation");
     constexpr int NB = N / R;
@@ -161,10 +207,11 @@ template <int N> struct WaveFft {
       const int k = j % NS;
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        float2 w = tw[TOFF + (r - 1) * NS + k];
-        u[q][r] = cmul(u[q][r], make_float2(w.x, -sgn * w.y));
+        // the table holds the forward twiddles exp(-2 pi i m/N); the inverse conjugates them
+        const float2 w = tw[TOFF + (r - 1) * NS + k];
+        u[q][r] = S < 0 ? cmul(u[q][r], w) : cmulc(u[q][r], w);
       }
-      RegDft<R>::run(u[q], sgn);
+      RegDft<R, S>::run(u[q]);
     }
     if constexpr (LAST) {
       // outputs j + r*NS with NS = N/R: lane holds X[lane + 64*(q + r*E/R)]
@@ -181,20 +228,28 @@ template <int N> struct WaveFft {
         for (int r = 0; r < R; ++r) scr[pad(base + r * NS)] = u[q][r];
       }
       __builtin_amdgcn_wave_barrier();
-      pass<NS * R>(v, scr, tw, lane, sgn);
+      pass<NS * R, S>(v, scr, tw, lane);
     }
   }
 
   // Transforms v in place (see the convention above); scr: SCR wave-private LDS elements;
   // tw: the TW-entry pass-major table from build_twiddles; inv: kernel exp(+...), unscaled.
-  __device__ __forceinline__ static void run(float2 (&v)[E], float2 *scr, const float2 *tw, int lane,
-                                             bool inv) {
-    const float sgn = inv ? 1.0f : -1.0f;
-    RegDft<E>::run(v, sgn);  // pass 1: butterfly j = lane over x[lane + 64 r]
+  template <bool INV>
+  __device__ __forceinline__ static void run(float2 (&v)[E], float2 *scr, const float2 *tw, int lane) {
+    constexpr int S = INV ? 1 : -1;
+    RegDft<E, S>::run(v);  // pass 1: butterfly j = lane over x[lane + 64 r]
 #pragma unroll
     for (int r = 0; r < E; ++r) scr[pad(lane * E + r)] = v[r];
     __builtin_amdgcn_wave_barrier();
-    pass<E>(v, scr, tw, lane, sgn);
+    pass<E, S>(v, scr, tw, lane);
+  }
+  // run-time direction (a wave-uniform branch between the two instantiations)
+  __device__ __forceinline__ static void run(float2 (&v)[E], float2 *scr, const float2 *tw, int lane,
+                                             bool inv) {
+    if (inv)
+      run<true>(v, scr, tw, lane);
+    else
+      run<false>(v, scr, tw, lane);
   }
 };
 

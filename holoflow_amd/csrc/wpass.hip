@@ -19,6 +19,11 @@ namespace {
 // BLK_TO_MOM5: moments plus the two band-split M0 images (5 sums instead of 3)
 enum Kind { NAT_TO_BLK = 0, BLK_TO_MOM = 1, BLK_MID = 2, BLK_TO_NAT = 3, BLK_TO_MOM5 = 4 };
 constexpr bool is_mom(int k) { return k == BLK_TO_MOM || k == BLK_TO_MOM5; }
+// Transform direction each kind is compiled for: the row pass out of natural data is
+// forward (it feeds the column pass), the row passes that read the blocked spectrum back
+// (angular spectrum) are inverse, and every column pass is forward (BLK_MID runs forward,
+// transfer function, inverse).  The launchers reject a PassArgs::inv that disagrees.
+constexpr bool row_inv(int k) { return k != NAT_TO_BLK; }
 
 __device__ __forceinline__ long long blk_index(int y, int x, int H, int C) {
   return (long long)(x / C) * H * C + (long long)y * C + (x % C);
@@ -123,14 +128,14 @@ __global__ __launch_bounds__(64 * WROW_WAVES) void k_wrow(PassArgs a) {
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = nxt[m];
         if (p + 1 < a.planes) load(nxt, p + 1);
-        F::run(v, ws, twl, lane, a.inv != 0);
+        F::template run<row_inv(KIND)>(v, ws, twl, lane);
         acc.add(v, a.mom, p);
       }
     } else {  // 2048-point rows: no register room for a prefetch next to 5*E sums
       for (int p = 0; p < a.planes; ++p) {
         float2 v[E];
         load(v, p);
-        F::run(v, ws, twl, lane, a.inv != 0);
+        F::template run<row_inv(KIND)>(v, ws, twl, lane);
         acc.add(v, a.mom, p);
       }
     }
@@ -147,7 +152,7 @@ __global__ __launch_bounds__(64 * WROW_WAVES) void k_wrow(PassArgs a) {
     const long long p = blockIdx.y;
     float2 v[E];
     load(v, p);
-    F::run(v, ws, twl, lane, a.inv != 0);
+    F::template run<row_inv(KIND)>(v, ws, twl, lane);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int x = lane + 64 * m;
@@ -210,7 +215,7 @@ __global__ __launch_bounds__(is_mom(KIND) ? 256 * wcol_subsets(N) : 256) void k_
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = nxt[m];
         if (p + S < a.planes) load(nxt, p + S);
-        F::run(v, ws, twl, lane, a.inv != 0);
+        F::template run<false>(v, ws, twl, lane);
         acc.add(v, a.mom, p);
       }
     }
@@ -251,15 +256,15 @@ __global__ __launch_bounds__(is_mom(KIND) ? 256 * wcol_subsets(N) : 256) void k_
     float2 v[E];
     load(v, p);
     if (KIND == BLK_MID) {
-      F::run(v, ws, twl, lane, false);
+      F::template run<false>(v, ws, twl, lane);
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.mid_tf[tile + (long long)(lane + 64 * m) * C + c]);
-      F::run(v, ws, twl, lane, true);
+      F::template run<true>(v, ws, twl, lane);
 #pragma unroll
       for (int m = 0; m < E; ++m)
         reinterpret_cast<float2 *>(a.out)[p * plane_elems + tile + (long long)(lane + 64 * m) * C + c] = v[m];
     } else {
-      F::run(v, ws, twl, lane, a.inv != 0);
+      F::template run<false>(v, ws, twl, lane);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int y = lane + 64 * m;
@@ -282,6 +287,7 @@ __global__ __launch_bounds__(is_mom(KIND) ? 256 * wcol_subsets(N) : 256) void k_
 bool wave_pass_supported(int n) { return n == 256 || n == 512 || n == 768 || n == 1024 || n == 2048; }
 
 template <int N, int KIND> static int launch_wrow_n(const PassArgs &a, hipStream_t st) {
+  if ((a.inv != 0) != row_inv(KIND)) return -5;
   const unsigned gx = (unsigned)((a.H + WROW_WAVES - 1) / WROW_WAVES);
   const unsigned gy = is_mom(KIND) ? 1u : (unsigned)a.planes;
   hipLaunchKernelGGL((k_wrow<N, KIND>), dim3(gx, gy), dim3(64 * WROW_WAVES), 0, st, a);
@@ -289,6 +295,7 @@ template <int N, int KIND> static int launch_wrow_n(const PassArgs &a, hipStream
 }
 
 template <int N, int KIND> static int launch_wcol_n(const PassArgs &a, hipStream_t st) {
+  if (a.inv != 0) return -5;
   const unsigned gx = (unsigned)(a.W / a.blk_c);
   const unsigned gy = is_mom(KIND) ? 1u : (unsigned)a.planes;
   const unsigned nt = is_mom(KIND) ? 256 * wcol_subsets(N) : 256;
