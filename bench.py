@@ -240,7 +240,7 @@ def main():
 
     path_bytes = T * npix * 2 + 2 * nb * npix * 8 + 3 * npix * 4
     path_gbs = path_bytes * value / 1e9
-    traffic = measured_traffic("k_dft_row")
+    traffic = measured_traffic("k_front")
 
     # ---- per-window GPU latency (events around each window's kernels)
     lat = []
@@ -330,7 +330,7 @@ def main():
                    "frames": args.frames, "windows_per_step": n_windows, "precision": "fp32"},
         "roofline": {"bound": "hbm", "achieved": round(front_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(front_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_dft_row<1024,2,0,false> (time DFT fused with the row FFT)",
+                     "kernel": "k_front<1024,2,0,64> (MFMA time DFT fused with the row FFT)",
                      "kernel_us": round(front_ms * 1e3, 2), "algorithmic_bytes_per_launch": front_bytes},
         "kernels": {"front_us": round(front_ms * 1e3, 2), "front_gbs": round(front_gbs, 1),
                     "back_us": round(back_ms * 1e3, 2), "back_gbs": round(back_gbs, 1),

@@ -174,9 +174,11 @@ __global__ __launch_bounds__(64 * WROW_WAVES) void k_wrow(PassArgs a) {
 
 // ------------------------------------------------------------------------- column pass
 // Plane subsets per column tile (waves = 4 * subsets).  A wave keeps its column (E
-// values), the next plane's prefetch and 5*E moment sums in registers, which allows 2
-// waves per SIMD up to 1024 rows and 1 for 2048.
-constexpr int wcol_subsets(int n) { return n >= 2048 ? 1 : 2; }
+// values) and 5*E moment sums in registers.  Three subsets (12 waves, 3 per SIMD) without
+// a register prefetch measured faster at 1024 rows than two subsets with the next plane
+// prefetched into registers; 2048-row columns leave room for one subset only.
+constexpr int wcol_subsets(int n) { return n >= 2048 ? 1 : 3; }
+constexpr bool wcol_prefetch(int n) { return n >= 2048 ? false : wcol_subsets(n) < 3; }
 
 template <int N, int KIND>
 __global__ __launch_bounds__(is_mom(KIND) ? 256 * wcol_subsets(N) : 256) void k_wcol(PassArgs a) {
@@ -208,13 +210,20 @@ __global__ __launch_bounds__(is_mom(KIND) ? 256 * wcol_subsets(N) : 256) void k_
     constexpr int NACC = KIND == BLK_TO_MOM5 ? 5 : 3;
     Moments<E, NACC> acc;
     acc.zero();
-    if (active) {
+    if (active && wcol_prefetch(N)) {
       float2 v[E], nxt[E];
       if (sub < a.planes) load(nxt, sub);
       for (int p = sub; p < a.planes; p += S) {
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = nxt[m];
         if (p + S < a.planes) load(nxt, p + S);
+        F::template run<false>(v, ws, twl, lane);
+        acc.add(v, a.mom, p);
+      }
+    } else if (active) {  // more waves per CU instead of a register prefetch
+      for (int p = sub; p < a.planes; p += S) {
+        float2 v[E];
+        load(v, p);
         F::template run<false>(v, ws, twl, lane);
         acc.add(v, a.mom, p);
       }
