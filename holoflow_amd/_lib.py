@@ -205,6 +205,27 @@ SIGNATURES.update({
     "hfx_serve_workload": (C.c_int, [C.POINTER(WorkloadDesc), C.POINTER(WorkloadResult)]),
 })
 
+
+class HolofileInfo(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("bits_per_pixel", C.c_uint16),
+                ("version", C.c_uint16), ("frame_count", C.c_uint32), ("data_size", C.c_uint64),
+                ("frame_bytes", C.c_uint64), ("has_footer", C.c_int32)]
+
+
+SIGNATURES.update({
+    # include/hfx_holofile.h
+    "hfx_holofile_open": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "hfx_holofile_close": (C.c_int, [C.c_void_p]),
+    "hfx_holofile_get_info": (C.c_int, [C.c_void_p, C.POINTER(HolofileInfo)]),
+    "hfx_holofile_footer": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
+    "hfx_holofile_seek": (C.c_int, [C.c_void_p, C.c_int64]),
+    "hfx_holofile_tell": (C.c_int64, [C.c_void_p]),
+    "hfx_holofile_read": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
+    "hfx_holofile_read_to_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    "hfx_holofile_write": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint16, C.c_void_p, C.c_int64,
+                                     C.c_char_p]),
+})
+
 _lib = None
 _lock = threading.Lock()
 

@@ -15,18 +15,23 @@
 //   Arange, Multiply            arange.cu:61-200, multiply.cu:69-92
 //   FFTShiftNp                  fftshift.cu:110-260
 //   DopplerMoments (fused)      the whole chain graph_builder.cc:245-305, 522-566, 604-647
+//   Holofile (source)           sources/holofile.cc:244-434
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <mutex>
 #include <numeric>
+#include <optional>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "curaii/hip.hh"
 #include "hfx.h"
+#include "hfx_holofile.h"
 #include "hfx_nodes.h"
 #include "holoflow/core/registry.hh"
 
@@ -1442,6 +1447,200 @@ class DualReaderBatchQueueFactory : public IAsyncTaskFactory {
 
 }  // namespace
 
+// ================================================================== Holofile (source)
+// Frame source over a .holo recording (sources/holofile.cc:30-434) through libhfx's reader
+// (include/hfx_holofile.h).  Settings {path, load_kind Live|CPUCached|GPUCached,
+// start_frame, end_frame, batch_size, max_fps?, keep_cursor=true}; output [batch_size, H, W]
+// U8 or U16 (8 or 16 bits per pixel).  Live reads each batch from the file into the
+// caller's host buffer; CPUCached / GPUCached preload frames [start_frame, end_frame) into
+// pinned host / device memory at create and hand out views into that cache (owned output).
+// A batch that would pass end_frame wraps the cursor back to start_frame.  max_fps paces
+// batches at batch_size / max_fps seconds apart.
+//
+// One deliberate difference: the reference's cached modes index the cache at
+// frame_idx * frame_bytes although the cache starts at start_frame (holofile.cc:223 against
+// :294, :332-337), which reads past the cache whenever start_frame > 0; here the view is at
+// (frame_idx - start_frame) * frame_bytes, the frame the cursor names.
+enum class LoadKind { Live, CPUCached, GPUCached };
+
+struct HolofileSettings {
+  std::string path;
+  LoadKind load_kind = LoadKind::Live;
+  long long start_frame = 0, end_frame = 0, batch_size = 0;
+  std::optional<double> max_fps;
+  bool keep_cursor = true;
+};
+
+HolofileSettings holofile_settings(const Json &j) {
+  HolofileSettings s;
+  s.path = j.at("path").str();
+  const std::string &lk = j.at("load_kind").str();
+  if (lk == "Live") s.load_kind = LoadKind::Live;
+  else if (lk == "CPUCached") s.load_kind = LoadKind::CPUCached;
+  else if (lk == "GPUCached") s.load_kind = LoadKind::GPUCached;
+  else throw std::invalid_argument("Invalid LoadKind string: " + lk);
+  s.start_frame = j.at("start_frame").integer();
+  s.end_frame = j.at("end_frame").integer();
+  s.batch_size = j.at("batch_size").integer();
+  if (j.contains("max_fps") && !j.at("max_fps").is_null()) s.max_fps = (double)j.at("max_fps").integer();
+  s.keep_cursor = j.get_bool("keep_cursor", true);
+  return s;
+}
+
+struct HolofileCloser {
+  void operator()(hfx_holofile *f) const { (void)hfx_holofile_close(f); }
+};
+using HolofileReader = std::unique_ptr<hfx_holofile, HolofileCloser>;
+
+HolofileReader open_holofile(const std::string &path, hfx_holofile_info &info) {
+  hfx_holofile *f = nullptr;
+  if (hfx_holofile_open(path.c_str(), &f) != HFX_OK) throw std::runtime_error(hfx_last_error());
+  HolofileReader r(f);
+  if (hfx_holofile_get_info(f, &info) != HFX_OK) throw std::runtime_error(hfx_last_error());
+  return r;
+}
+
+void holofile_call(hfx_status rc) {
+  if (rc == HFX_OK) return;
+  const std::string msg = hfx_last_error();
+  if (rc == HFX_EOF) throw std::out_of_range(msg);
+  throw HfxError(rc, msg);
+}
+
+class HolofileTask : public ISyncTask {
+ public:
+  using Clock = std::chrono::steady_clock;
+  HolofileSettings settings;
+  HolofileReader reader;
+  hfx_holofile_info header{};
+  long long frame_idx = 0;
+  TDesc odesc;
+  std::optional<Clock::time_point> next_batch_start;
+  std::byte *buf = nullptr;  // the cache of frames [start_frame, end_frame), cached kinds only
+  curaii::unique_host_ptr<std::byte> h_buf;
+  curaii::unique_device_ptr<std::byte> d_buf;
+
+  // Fixed-rate pacing (holofile.cc:153-183): wait for the batch's slot, then advance the
+  // slot by one period, skipping periods already in the past.
+  bool pace(SyncCtx &ctx) {
+    if (!settings.max_fps) {
+      next_batch_start.reset();
+      return true;
+    }
+    auto period = std::chrono::duration_cast<Clock::duration>(
+        std::chrono::duration<double>((double)settings.batch_size / *settings.max_fps));
+    if (period <= Clock::duration::zero()) period = Clock::duration{1};
+    if (!next_batch_start) next_batch_start = Clock::now();
+    while (Clock::now() < *next_batch_start) {
+      if (ctx.cancelled && ctx.cancelled->load(std::memory_order_relaxed)) return false;
+      std::this_thread::sleep_for(std::min<Clock::duration>(*next_batch_start - Clock::now(), std::chrono::milliseconds(1)));
+    }
+    do {
+      *next_batch_start += period;
+    } while (*next_batch_start < Clock::now());
+    return true;
+  }
+
+  void release_output(int index) override {
+    if (settings.load_kind == LoadKind::Live) throw std::logic_error("Cannot release output for Live load kind");
+    if (index != 0) throw std::out_of_range("Holofile task has only one output at index 0");
+  }
+
+  OpResult execute(SyncCtx &ctx) override {
+    if (!pace(ctx)) return OpResult::Cancelled;
+    if (frame_idx + settings.batch_size > settings.end_frame) {
+      if (settings.load_kind == LoadKind::Live) holofile_call(hfx_holofile_seek(reader.get(), settings.start_frame));
+      frame_idx = settings.start_frame;
+    }
+    if (settings.load_kind == LoadKind::Live) {
+      holofile_call(hfx_holofile_read(reader.get(), ctx.outputs[0].data(), settings.batch_size));
+    } else {
+      auto &st = storage_access().owned_output_storage(0);
+      st = Storage{odesc.mem_loc, odesc.num_bytes(),
+                   buf + (size_t)(frame_idx - settings.start_frame) * (size_t)header.frame_bytes};
+      ctx.outputs[0] = TView{odesc, &st};
+    }
+    frame_idx += settings.batch_size;
+    return OpResult::Ok;
+  }
+};
+
+class HolofileFactory : public ISyncTaskFactory {
+ public:
+  InferResult infer(std::span<const TDesc> in, const Json &j) const override {
+    const char *who = "HolofileFactory";
+    const auto s = holofile_settings(j);
+    hfx_holofile_info h{};
+    (void)open_holofile(s.path, h);
+    check(in.empty(), who, "Holofile task must have no inputs");
+    check(s.start_frame >= 0, who, "Invalid frame range");
+    check(s.start_frame < s.end_frame, who, "Invalid frame range");
+    check(s.batch_size > 0, who, "Batch size must be positive");
+    check(!s.max_fps || *s.max_fps > 0, who, "max_fps must be positive when provided");
+    check(s.end_frame <= (long long)h.frame_count, who, "end_frame exceeds total frames in file");
+    check(s.batch_size <= s.end_frame - s.start_frame, who, "Batch size exceeds available frames");
+    check(h.bits_per_pixel == 8 || h.bits_per_pixel == 16, who,
+          "Unsupported bits_per_pixel: " + std::to_string(h.bits_per_pixel));
+    TDesc out({(size_t)s.batch_size, (size_t)h.height, (size_t)h.width}, h.bits_per_pixel == 8 ? DType::U8 : DType::U16,
+              s.load_kind == LoadKind::GPUCached ? MemLoc::Device : MemLoc::Host);
+    InferResult r = sync_result({}, {out});
+    r.owned_outputs = {s.load_kind != LoadKind::Live};
+    return r;
+  }
+  std::unique_ptr<ISyncTask> create(std::span<const TDesc> in, const Json &j, const SyncCreateCtx &ctx) const override {
+    const auto r = infer(in, j);
+    auto t = std::make_unique<HolofileTask>();
+    t->settings = holofile_settings(j);
+    t->reader = open_holofile(t->settings.path, t->header);
+    holofile_call(hfx_holofile_seek(t->reader.get(), t->settings.start_frame));
+    const long long frames = t->settings.end_frame - t->settings.start_frame;
+    const size_t bytes = (size_t)frames * (size_t)t->header.frame_bytes;
+    if (t->settings.load_kind == LoadKind::CPUCached) {
+      t->h_buf = curaii::make_unique_host_ptr<std::byte>(bytes);
+      t->buf = t->h_buf.get();
+      holofile_call(hfx_holofile_read(t->reader.get(), t->buf, frames));
+    } else if (t->settings.load_kind == LoadKind::GPUCached) {
+      t->d_buf = curaii::make_unique_device_ptr<std::byte>(bytes);
+      t->buf = t->d_buf.get();
+      holofile_call(hfx_holofile_read_to_device(t->reader.get(), t->buf, frames, hs(ctx.stream)));
+    }
+    t->frame_idx = t->settings.start_frame;
+    t->odesc = r.output_descs[0];
+    return t;
+  }
+  // Same file and load kind (and, for the cached kinds, the same frame range) keep the
+  // reader and the cache; keep_cursor keeps the cursor when it stays inside the range
+  // (holofile.cc:361-431).
+  std::unique_ptr<ISyncTask> update(std::unique_ptr<ISyncTask> old, std::span<const TDesc> in, const Json &j,
+                                    const SyncCreateCtx &ctx) const override {
+    auto *o = dynamic_cast<HolofileTask *>(old.get());
+    if (!o) return create(in, j, ctx);
+    const auto r = infer(in, j);
+    const auto s = holofile_settings(j);
+    const bool live = s.load_kind == LoadKind::Live;
+    const bool bounds_same = o->settings.start_frame == s.start_frame && o->settings.end_frame == s.end_frame;
+    const bool pace_same = o->settings.batch_size == s.batch_size && o->settings.max_fps == s.max_fps;
+    if (!(o->settings.path == s.path && o->settings.load_kind == s.load_kind && (live || bounds_same)))
+      return create(in, j, ctx);
+    auto t = std::make_unique<HolofileTask>();
+    t->settings = s;
+    t->reader = std::move(o->reader);
+    t->header = o->header;
+    t->buf = o->buf;
+    t->h_buf = std::move(o->h_buf);
+    t->d_buf = std::move(o->d_buf);
+    t->frame_idx = s.start_frame;
+    if (s.keep_cursor) {
+      t->frame_idx = o->frame_idx;
+      if (t->frame_idx < s.start_frame || t->frame_idx >= s.end_frame) t->frame_idx = s.start_frame;
+    }
+    if (live && t->frame_idx != o->frame_idx) holofile_call(hfx_holofile_seek(t->reader.get(), t->frame_idx));
+    t->odesc = r.output_descs[0];
+    if (pace_same) t->next_batch_start = o->next_batch_start;
+    return t;
+  }
+};
+
 void register_hip_nodes(Registry &r) {
   r.register_async("BatchQueue", std::make_unique<BatchQueueFactory>());
   r.register_async("SlidingAverage", std::make_unique<SlidingAverageFactory>());
@@ -1462,6 +1661,7 @@ void register_hip_nodes(Registry &r) {
   r.register_sync("Multiply", std::make_unique<MultiplyFactory>());
   r.register_sync("FFTShiftNp", std::make_unique<FFTShiftFactory>());
   r.register_sync("DopplerMoments", std::make_unique<DopplerMomentsFactory>());
+  r.register_sync("Holofile", std::make_unique<HolofileFactory>());
 }
 
 }  // namespace holoflow::core
@@ -1623,7 +1823,7 @@ hfx_status hfx_node_update(hfx_node *n, const char *settings_json, const char *i
   });
 }
 
-hfx_status hfx_node_execute(hfx_node *n, void *const *inputs, int32_t n_inputs, void *const *outputs,
+hfx_status hfx_node_execute(hfx_node *n, void *const *inputs, int32_t n_inputs, void **outputs,
                             int32_t n_outputs) {
   if (!n || !n->sync) {
     hfx::set_last_error("not a synchronous node");
@@ -1635,16 +1835,21 @@ hfx_status hfx_node_execute(hfx_node *n, void *const *inputs, int32_t n_inputs, 
     std::vector<Storage> st(n_inputs + n_outputs);
     std::vector<TView> in(n_inputs), out(n_outputs);
     for (int i = 0; i < n_inputs; ++i) {
-      st[i] = Storage{MemLoc::Device, n->in_descs[i].num_bytes(), static_cast<std::byte *>(inputs[i])};
+      st[i] = Storage{n->in_descs[i].mem_loc, n->in_descs[i].num_bytes(), static_cast<std::byte *>(inputs[i])};
       in[i] = TView{n->in_descs[i], &st[i]};
     }
+    const auto &od = n->info.output_descs;
     for (int i = 0; i < n_outputs; ++i) {
-      st[n_inputs + i] = Storage{MemLoc::Device, n->info.output_descs[i].num_bytes(), static_cast<std::byte *>(outputs[i])};
-      out[i] = TView{n->info.output_descs[i], &st[n_inputs + i]};
+      // an output the task owns is a view it hands back; otherwise caller storage
+      const bool owned = i < (int)n->info.owned_outputs.size() && n->info.owned_outputs[i];
+      st[n_inputs + i] = Storage{od[i].mem_loc, od[i].num_bytes(), owned ? nullptr : static_cast<std::byte *>(outputs[i])};
+      out[i] = TView{od[i], &st[n_inputs + i]};
     }
     std::atomic<bool> cancelled{false};
     SyncCtx ctx{in, out, &cancelled};
     if (n->sync->execute(ctx) != OpResult::Ok) throw std::runtime_error("execute did not return Ok");
+    for (int i = 0; i < n_outputs; ++i)
+      if (i < (int)n->info.owned_outputs.size() && n->info.owned_outputs[i]) outputs[i] = out[i].data();
   });
 }
 
@@ -1711,8 +1916,11 @@ hfx_status hfx_node_pop(hfx_node *n, void **outputs, int32_t n_outputs) {
 }
 
 hfx_status hfx_node_release_output(hfx_node *n, int32_t index) {
-  if (!n || !n->async) return HFX_E_INVALID;
-  return guarded([&] { n->async->release_output(index); });
+  if (!n || (!n->async && !n->sync)) return HFX_E_INVALID;
+  return guarded([&] {
+    if (n->async) n->async->release_output(index);
+    else n->sync->release_output(index);
+  });
 }
 
 hfx_status hfx_node_destroy(hfx_node *n) {

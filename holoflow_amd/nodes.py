@@ -98,34 +98,49 @@ class Node:
         return bool(reused.value)
 
     def execute(self, *tensors: torch.Tensor) -> List[torch.Tensor]:
-        """Runs the task on device tensors whose descriptors match the node's inputs.
+        """Runs the task on tensors whose descriptors match the node's inputs.
 
         Outputs aliased to an input (in_place) are returned as views of that input's
-        storage; the others are freshly allocated contiguous tensors."""
+        storage; outputs the node owns (a cached Holofile source's views into its cache) are
+        copied out and released; the others are freshly allocated contiguous tensors, on the
+        CPU for host-memory outputs."""
         if len(tensors) != len(self.inputs):
             raise ValueError(f"{self.kind}: expected {len(self.inputs)} inputs, got {len(tensors)}")
         for t, d in zip(tensors, self.inputs):
             if desc_of(t)["shape"] != d["shape"] or _NAMES[t.dtype] != d["dtype"]:
                 raise ValueError(f"{self.kind}: tensor {tuple(t.shape)} {t.dtype} does not match {d}")
         aliased = {o: i for i, o in self.info["in_place"]}
+        owned = self.info["owned_outputs"]
         outs, optrs = [], []
         for k, od in enumerate(self.output_descs):
             if k in aliased:
                 src = tensors[aliased[k]]
                 outs.append(_storage_view(src, od))
                 optrs.append(src.untyped_storage().data_ptr())
+            elif k < len(owned) and owned[k]:
+                outs.append(None)
+                optrs.append(None)
             else:
-                t = torch.empty(od["shape"], dtype=DTYPES[od["dtype"]], device="cuda")
+                dev = "cpu" if od["mem_loc"] == "Host" else "cuda"
+                t = torch.empty(od["shape"], dtype=DTYPES[od["dtype"]], device=dev)
                 outs.append(t)
                 optrs.append(t.data_ptr())
         iptrs = [t.untyped_storage().data_ptr() for t in tensors]
         ia = (C.c_void_p * max(1, len(iptrs)))(*iptrs)
         oa = (C.c_void_p * max(1, len(optrs)))(*optrs)
         check(lib().hfx_node_execute(self._h, ia, len(iptrs), oa, len(optrs)))
+        if any(o is None for o in outs):
+            for k, od in enumerate(self.output_descs):
+                if outs[k] is None:
+                    outs[k] = _copy_owned(oa[k], od, self._stream)
+            torch.cuda.synchronize()
+            for k, o in enumerate(owned):
+                if o:
+                    check(lib().hfx_node_release_output(self._h, k))
         return outs
 
     def close(self):
-        if self._h:
+        if self._h and lib is not None:      # module globals are gone at interpreter exit
             lib().hfx_node_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -138,6 +153,18 @@ def _copy_raw(src_ptr: int, d: Dict, dst_ptr: int, stream) -> None:
     strides = (C.c_int64 * nd)(*(d["strides"] or [_ESZ[d["dtype"]]]))
     check(lib().hfx_strided_copy(C.c_void_p(src_ptr), shape, strides, d["offset"], len(d["shape"]) or 1,
                                  _ESZ[d["dtype"]], C.c_void_p(dst_ptr), 0, 0, stream))
+
+
+def _copy_owned(ptr: int, od: Dict, stream) -> torch.Tensor:
+    """A new tensor holding the contiguous output a node owns at `ptr`."""
+    if od["mem_loc"] == "Host":
+        n = int(np.prod(od["shape"])) * _ESZ[od["dtype"]]
+        host = np.empty(n, np.uint8)
+        C.memmove(host.ctypes.data, ptr, n)
+        return torch.from_numpy(host).view(DTYPES[od["dtype"]]).reshape(od["shape"])
+    out = torch.empty(od["shape"], dtype=DTYPES[od["dtype"]], device="cuda")
+    _copy_raw(ptr, desc(od["shape"], od["dtype"]), out.data_ptr(), stream)
+    return out
 
 
 class AsyncNode(Node):
@@ -176,15 +203,7 @@ class AsyncNode(Node):
             if i in flags:
                 outs.append(int(flags[i][0]))
                 continue
-            if od["mem_loc"] == "Host":       # a host-memory queue: plain host copy
-                n = int(np.prod(od["shape"])) * _ESZ[od["dtype"]]
-                host = np.empty(n, np.uint8)
-                C.memmove(host.ctypes.data, ptrs[i], n)
-                outs.append(torch.from_numpy(host).view(DTYPES[od["dtype"]]).reshape(od["shape"]))
-                continue
-            out = torch.empty(od["shape"], dtype=DTYPES[od["dtype"]], device="cuda")
-            _copy_raw(ptrs[i], desc(od["shape"], od["dtype"]), out.data_ptr(), self._stream)
-            outs.append(out)
+            outs.append(_copy_owned(ptrs[i], od, self._stream))
         torch.cuda.synchronize()
         for i, o in enumerate(owned):
             if o:

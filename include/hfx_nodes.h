@@ -47,8 +47,10 @@ hfx_status hfx_node_update(hfx_node *node, const char *settings_json, const char
                            hfx_stream_t stream, int32_t *reused);
 
 /* ISyncTask::execute (tasks.hh:182-190) with storage base pointers for the inferred inputs
- * and outputs; work is enqueued on the node's stream. */
-hfx_status hfx_node_execute(hfx_node *node, void *const *inputs, int32_t n_inputs, void *const *outputs,
+ * and outputs (host or device memory, as each descriptor's mem_loc says); work is enqueued
+ * on the node's stream.  For an output the node owns (infer's owned_outputs, e.g. a cached
+ * Holofile source) outputs[i] receives the pointer of the view it hands out instead. */
+hfx_status hfx_node_execute(hfx_node *node, void *const *inputs, int32_t n_inputs, void **outputs,
                             int32_t n_outputs);
 
 /* IAsyncTask owned slots and queue operations (tasks.hh:136-151, 208-220):
@@ -57,7 +59,8 @@ hfx_status hfx_node_execute(hfx_node *node, void *const *inputs, int32_t n_input
  * U8 validity flag), as storage base pointers.  pop: outputs[i] receives the storage
  * pointer of each output the node owns; for an output it does not own (DualReaderBatchQueue's
  * host U8 validity flag) the caller passes the buffer the node writes.  HFX_NOT_READY when
- * nothing can be popped.  release_output(index) releases one owned output. */
+ * nothing can be popped.  release_output(index) releases one owned output (of an async node,
+ * or of a sync node that owns its outputs). */
 hfx_status hfx_node_acquire_input(hfx_node *node, void **slot);
 hfx_status hfx_node_push(hfx_node *node, void *const *extra_inputs, int32_t n_extra);
 hfx_status hfx_node_pop(hfx_node *node, void **outputs, int32_t n_outputs);
