@@ -128,6 +128,14 @@ SIGNATURES = {
     "hfx_asm_create": (C.c_int, [_P(AsmDesc), _VP, _P(_VP)]),
     "hfx_asm_execute": (C.c_int, [_VP, _VP, _VP, _VP]),
     "hfx_asm_destroy": (C.c_int, [_VP]),
+    "hfx_filter2d_create": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                      C.c_int32, _VP, _P(_VP)]),
+    "hfx_filter2d_execute": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "hfx_filter2d_destroy": (C.c_int, [_VP]),
+    "hfx_flatfield_create": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_float, _VP, _P(_VP),
+                                       _P(C.c_int32)]),
+    "hfx_flatfield_execute": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "hfx_flatfield_destroy": (C.c_int, [_VP]),
     "hfx_weighted_mean": (C.c_int, [_VP, C.c_int32, C.c_int64, C.c_int64, _VP, C.c_uint32, _VP, _VP]),
     "hfx_strided_copy": (C.c_int, [_VP, _P(C.c_int64), _P(C.c_int64), C.c_int64, C.c_int32, C.c_int32, _VP,
                                    C.c_int64, C.c_int64, _VP]),

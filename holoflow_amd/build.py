@@ -17,7 +17,7 @@ ROOT = os.path.dirname(HERE)
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhfx.so")
 
-SOURCES = ["tables.hip", "dft.hip", "fft2d.hip", "wpass.hip", "ops.hip", "synth.hip", "api.hip", "nodes.cpp", "serve.cpp", "holofile.cpp"]
+SOURCES = ["tables.hip", "dft.hip", "fft2d.hip", "wpass.hip", "ops.hip", "synth.hip", "filters.hip", "api.hip", "nodes.cpp", "serve.cpp", "holofile.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

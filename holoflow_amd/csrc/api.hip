@@ -4,8 +4,10 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -116,6 +118,30 @@ struct hfx_asm {
   int OH = 0, OW = 0, blk_c = 1;
   DevArena arena;
   float2 *tw_w = nullptr, *tw_h = nullptr, *tf = nullptr, *ws = nullptr;
+};
+
+// A spectral filter: forward row pass -> column pass with the table between its forward
+// and inverse transforms -> inverse row pass (Filter2D, and Flatfield's Fourier form).
+struct SpectralFilter {
+  int64_t batch = 0;
+  int H = 0, W = 0, blk_c = 1;
+  DevArena arena;
+  float2 *tw_w = nullptr, *tw_h = nullptr, *tf = nullptr, *ws = nullptr;
+};
+
+struct hfx_filter2d {
+  SpectralFilter sf;
+  int in_real = 0;
+};
+
+struct hfx_flatfield {
+  int64_t batch = 0;
+  int H = 0, W = 0;
+  bool use_fft = false;
+  int rx = 0, ry = 0;
+  DevArena arena;
+  float *kx = nullptr, *ky = nullptr, *tmp = nullptr;
+  SpectralFilter sf;
 };
 
 extern "C" {
@@ -603,6 +629,166 @@ hfx_status hfx_asm_execute(hfx_asm *a, const void *in, void *out, hfx_stream_t s
 
 hfx_status hfx_asm_destroy(hfx_asm *a) {
   delete a;
+  return HFX_OK;
+}
+
+}  // extern "C"
+
+static hfx_status spectral_setup(SpectralFilter &f, int64_t batch, int H, int W, hipStream_t st) {
+  if (batch <= 0 || H <= 0 || W <= 0) return fail(HFX_E_INVALID, "invalid shape");
+  if (batch > 65535) return fail(HFX_E_UNSUPPORTED, "more than 65535 planes");
+  f.batch = batch;
+  f.H = H;
+  f.W = W;
+  f.blk_c = hfx::pick_blk_c(W, H);
+  if (!hfx::row_pass_supported(W) || !hfx::col_pass_supported(H, f.blk_c))
+    return fail(HFX_E_UNSUPPORTED, "FFT size must be one of the compiled 2^a*3^b lengths (2..2048)");
+  HFX_TRY(f.arena.alloc(&f.tw_w, W));
+  HFX_TRY(f.arena.alloc(&f.tw_h, H));
+  HFX_TRY(f.arena.alloc(&f.tf, (size_t)H * W));
+  HFX_TRY(f.arena.alloc(&f.ws, (size_t)batch * H * W));
+  hfx::launch_fft_twiddles(f.tw_w, W, st);
+  hfx::launch_fft_twiddles(f.tw_h, H, st);
+  return HFX_OK;
+}
+
+static hfx_status spectral_run(const SpectralFilter &f, const void *in, bool in_real, void *out, bool out_real,
+                               hipStream_t st) {
+  const int B = (int)f.batch;
+  hfx::PassArgs r = base_pass(f.H, f.W, B, f.blk_c);
+  r.in = in;
+  r.in_mode = in_real ? hfx::IN_NAT_R : hfx::IN_NAT_C;
+  r.out = f.ws;
+  r.out_mode = hfx::OUT_BLK;
+  r.tw = f.tw_w;
+  if (hfx::launch_row_pass(r, st) != 0) return fail(HFX_E_HIP, "row pass launch failed");
+  hfx::PassArgs c = base_pass(f.H, f.W, B, f.blk_c);
+  c.in = f.ws;
+  c.in_mode = hfx::IN_BLK;
+  c.mid_tf = f.tf;
+  c.out = f.ws;
+  c.out_mode = hfx::OUT_BLK;
+  c.tw = f.tw_h;
+  if (hfx::launch_col_pass(c, st) != 0) return fail(HFX_E_HIP, "column pass launch failed");
+  hfx::PassArgs r2 = base_pass(f.H, f.W, B, f.blk_c);
+  r2.in = f.ws;
+  r2.in_mode = hfx::IN_BLK;
+  r2.inv = 1;
+  r2.out = out;
+  r2.out_mode = out_real ? hfx::OUT_NAT_RE : hfx::OUT_NAT_C;
+  r2.tw = f.tw_w;
+  if (hfx::launch_row_pass(r2, st) != 0) return fail(HFX_E_HIP, "inverse row pass launch failed");
+  return HFX_OK;
+}
+
+// Normalised Gaussian taps of radius ceil(3 sigma), in FP32 (flatfield.cu:73-94).
+static bool gaussian_taps(float sigma, std::vector<float> &k) {
+  const int radius = (int)std::ceil(3.0f * sigma);
+  if (radius <= 0) return false;
+  k.assign((size_t)(2 * radius + 1), 0.0f);
+  const float inv_two_sigma2 = 1.0f / (2.0f * sigma * sigma);
+  float sum = 0.0f;
+  for (int i = -radius; i <= radius; ++i) {
+    const float v = std::exp(-(float)(i * i) * inv_two_sigma2);
+    k[(size_t)(i + radius)] = v;
+    sum += v;
+  }
+  for (auto &v : k) v /= sum;
+  return true;
+}
+
+// A float as the reference's generated callback source spells it: printf "%f" (six
+// decimals), read back as a float literal (flatfield.cu:143-155).
+static float six_decimals(float v) {
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%f", (double)v);
+  return std::strtof(buf, nullptr);
+}
+
+extern "C" {
+
+hfx_status hfx_filter2d_create(int64_t batch, int32_t height, int32_t width, int32_t in_real, int32_t r_inner,
+                               int32_t r_outer, int32_t s_inner, int32_t s_outer, hfx_stream_t stream,
+                               hfx_filter2d **out) {
+  if (!out) return fail(HFX_E_INVALID, "null argument");
+  *out = nullptr;
+  if (r_inner < 0 || r_outer < 0 || s_inner < 0 || s_outer < 0)
+    return fail(HFX_E_INVALID, "radii and tapers must be non-negative");
+  auto f = std::make_unique<hfx_filter2d>();
+  f->in_real = in_real ? 1 : 0;
+  hipStream_t st = S(stream);
+  HFX_TRY(spectral_setup(f->sf, batch, height, width, st));
+  hfx::launch_filter2d_table(f->sf.tf, height, width, r_inner, r_outer, s_inner, s_outer, !f->in_real,
+                             f->sf.blk_c, st);
+  HFX_TRY_HIP(hipGetLastError());
+  HFX_TRY_HIP(hipStreamSynchronize(st));
+  *out = f.release();
+  return HFX_OK;
+}
+
+hfx_status hfx_filter2d_execute(hfx_filter2d *f, const void *in, void *out, hfx_stream_t stream) {
+  if (!f || !in || !out) return fail(HFX_E_INVALID, "null argument");
+  return spectral_run(f->sf, in, f->in_real != 0, out, f->in_real != 0, S(stream));
+}
+
+hfx_status hfx_filter2d_destroy(hfx_filter2d *f) {
+  delete f;
+  return HFX_OK;
+}
+
+hfx_status hfx_flatfield_create(int64_t batch, int32_t height, int32_t width, float sigma_y, float sigma_x,
+                                hfx_stream_t stream, hfx_flatfield **out, int32_t *uses_fft) {
+  if (!out) return fail(HFX_E_INVALID, "null argument");
+  *out = nullptr;
+  if (batch <= 0 || height <= 0 || width <= 0) return fail(HFX_E_INVALID, "invalid shape");
+  if (!(sigma_y > 0.0f) || !(sigma_x > 0.0f)) return fail(HFX_E_INVALID, "sigma must be positive");
+  std::vector<float> ky, kx;
+  if (!gaussian_taps(sigma_y, ky) || !gaussian_taps(sigma_x, kx))
+    return fail(HFX_E_INVALID, "sigma is too small to form a Gaussian kernel");
+  auto f = std::make_unique<hfx_flatfield>();
+  f->batch = batch;
+  f->H = height;
+  f->W = width;
+  f->ry = (int)(ky.size() / 2);
+  f->rx = (int)(kx.size() / 2);
+  // flatfield.cu:182-187: the Fourier form for kernels of 65 taps or more on 2-D planes
+  f->use_fft = height > 1 && width > 1 && std::max(ky.size(), kx.size()) >= 65;
+  hipStream_t st = S(stream);
+  if (f->use_fft) {
+    HFX_TRY(spectral_setup(f->sf, batch, height, width, st));
+    const float scale = six_decimals(1.0f / (float)(width * height));
+    hfx::launch_flatfield_table(f->sf.tf, height, width, six_decimals(sigma_y), six_decimals(sigma_x), scale,
+                                f->sf.blk_c, st);
+  } else {
+    if (!hfx::flatfield_spatial_fused(f->rx, f->ry)) {
+      HFX_TRY(f->arena.alloc(&f->tmp, (size_t)batch * height * width));
+    } else if (batch > 65535) {
+      return fail(HFX_E_UNSUPPORTED, "more than 65535 planes");
+    }
+    HFX_TRY(f->arena.alloc(&f->ky, ky.size()));
+    HFX_TRY(f->arena.alloc(&f->kx, kx.size()));
+    HFX_TRY_HIP(hipMemcpyAsync(f->ky, ky.data(), ky.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    HFX_TRY_HIP(hipMemcpyAsync(f->kx, kx.data(), kx.size() * sizeof(float), hipMemcpyHostToDevice, st));
+  }
+  HFX_TRY_HIP(hipGetLastError());
+  HFX_TRY_HIP(hipStreamSynchronize(st));
+  if (uses_fft) *uses_fft = f->use_fft ? 1 : 0;
+  *out = f.release();
+  return HFX_OK;
+}
+
+hfx_status hfx_flatfield_execute(hfx_flatfield *f, const float *in, float *out, hfx_stream_t stream) {
+  if (!f || !in || !out) return fail(HFX_E_INVALID, "null argument");
+  if (in == out) return fail(HFX_E_INVALID, "flatfield output must not alias its input");
+  hipStream_t st = S(stream);
+  if (f->use_fft) return spectral_run(f->sf, in, true, out, true, st);
+  if (hfx::launch_flatfield_spatial(in, out, f->tmp, f->batch, f->H, f->W, f->kx, f->rx, f->ky, f->ry, st) != 0)
+    return fail(HFX_E_HIP, "flatfield launch failed");
+  return HFX_OK;
+}
+
+hfx_status hfx_flatfield_destroy(hfx_flatfield *f) {
+  delete f;
   return HFX_OK;
 }
 

@@ -210,6 +210,8 @@ __global__ __launch_bounds__((Geo<N, COLS>::NT)) void k_pass(PassArgs a) {
             const long long o = p * plane_elems + (long long)y * a.W + x;
             if (a.out_mode == OUT_NAT_MAG) {
               reinterpret_cast<float *>(a.out)[o] = sqrtf(v.x * v.x + v.y * v.y);
+            } else if (a.out_mode == OUT_NAT_RE) {
+              reinterpret_cast<float *>(a.out)[o] = v.x;
             } else {
               if (a.post_lx) v = cmul(v, cmul(a.post_lx[x], a.post_ly[y]));
               reinterpret_cast<float2 *>(a.out)[o] = v;
@@ -292,7 +294,7 @@ int launch_row_pass(const PassArgs &a, hipStream_t st) {
     return dispatch<false, NAT_TO_BLK>(a, a.W, st);
   }
   if (a.out_mode == OUT_MOMENTS) return dispatch<false, BLK_TO_MOM>(a, a.W, st);
-  if (a.out_mode == OUT_NAT_C || a.out_mode == OUT_NAT_MAG) return dispatch<false, BLK_TO_NAT>(a, a.W, st);
+  if (a.out_mode == OUT_NAT_C || a.out_mode == OUT_NAT_MAG || a.out_mode == OUT_NAT_RE) return dispatch<false, BLK_TO_NAT>(a, a.W, st);
   return -4;
 }
 
@@ -304,7 +306,7 @@ int launch_col_pass(const PassArgs &a, hipStream_t st) {
     return dispatch<true, BLK_MID>(a, a.H, st);
   }
   if (a.out_mode == OUT_MOMENTS) return dispatch<true, BLK_TO_MOM>(a, a.H, st);
-  if (a.out_mode == OUT_NAT_C || a.out_mode == OUT_NAT_MAG) return dispatch<true, BLK_TO_NAT>(a, a.H, st);
+  if (a.out_mode == OUT_NAT_C || a.out_mode == OUT_NAT_MAG || a.out_mode == OUT_NAT_RE) return dispatch<true, BLK_TO_NAT>(a, a.H, st);
   return -4;
 }
 

@@ -22,6 +22,15 @@ void launch_asm_transfer(float2 *out, int H, int W, double lambda, double z, dou
                          int filt, double r_in, double r_out, double s_in, double s_out, int blk_c,
                          hipStream_t st);
 
+// ---- optional in-path stages (filters.hip)
+void launch_filter2d_table(float2 *out, int H, int W, int r_in, int r_out, int s_in, int s_out,
+                           int complex_mode, int blk_c, hipStream_t st);
+void launch_flatfield_table(float2 *out, int H, int W, float sigma_y, float sigma_x, float scale, int blk_c,
+                            hipStream_t st);
+bool flatfield_spatial_fused(int rx, int ry);
+int launch_flatfield_spatial(const float *in, float *out, float *tmp, long long planes, int H, int W,
+                             const float *kx, int rx, const float *ky, int ry, hipStream_t st);
+
 // ---- time DFT (dft.hip)
 struct DftLaunch {
   const uint16_t *ring;
@@ -44,7 +53,7 @@ int launch_time_dft_row(const DftLaunch &L, int H, int blk_c, int n_planes, cons
 
 // ---- 2-D FFT passes (fft2d.hip)
 enum InMode { IN_NAT_C = 0, IN_NAT_R = 1, IN_BLK = 2 };
-enum OutMode { OUT_BLK = 0, OUT_NAT_C = 1, OUT_NAT_MAG = 2, OUT_MOMENTS = 3 };
+enum OutMode { OUT_BLK = 0, OUT_NAT_C = 1, OUT_NAT_MAG = 2, OUT_MOMENTS = 3, OUT_NAT_RE = 4 };
 
 struct MomentArgs {
   float *out[5];       // M0, M1, M2, M0 low band, M0 high band (out[3..4] may be null)

@@ -11,6 +11,7 @@
 //   RFFT, FFT                   rfft.cu:148-297, fft.cu:234-386
 //   FresnelDiffraction          fresnel_diffraction.cu:386-668
 //   AngularSpectrum             angular_spectrum.cu:393-580
+//   Filter2D, Flatfield         filter2d.cu:359-497, flatfield.cu:323-451
 //   MeanAbs, Mean, Abs          mean_abs.cu:231-400, mean.cu:254-435, abs.cu:58-70
 //   Arange, Multiply            arange.cu:61-200, multiply.cu:69-92
 //   FFTShiftNp                  fftshift.cu:110-260
@@ -700,6 +701,138 @@ class AngularSpectrumFactory : public ISyncTaskFactory {
                                     const SyncCreateCtx &ctx) const override {
     auto *t = dynamic_cast<AsmTask *>(old.get());
     if (t && in.size() == 1 && in[0] == t->idesc && j == t->settings) {
+      t->st = ctx.stream;
+      return old;
+    }
+    return create(in, j, ctx);
+  }
+};
+
+// ================================================================== Filter2D / Flatfield
+// Optional in-path stages (filter2d.cu:359-497, flatfield.cu:323-451): settings keys,
+// checks and reuse rules as the reference's factories; the arithmetic runs through
+// hfx_filter2d_* / hfx_flatfield_* (2-D FFT passes, or the spatial blur).
+struct StageGeo {
+  int64_t batch;
+  int32_t H, W;
+};
+
+StageGeo stage_geo(const TDesc &d) {
+  const size_t r = d.rank();
+  return {(int64_t)prod(d.shape, 0, r - 2), (int32_t)d.shape[r - 2], (int32_t)d.shape[r - 1]};
+}
+
+class Filter2DTask : public ISyncTask {
+ public:
+  hipStream_t st;
+  Json settings;
+  TDesc idesc;
+  hfx_filter2d *plan = nullptr;
+  ~Filter2DTask() override {
+    if (plan) hfx_filter2d_destroy(plan);
+  }
+  OpResult execute(SyncCtx &c) override {
+    HFX_CHECK(hfx_filter2d_execute(plan, c.inputs[0].data(), c.outputs[0].data(), hs(st)));
+    return OpResult::Ok;
+  }
+};
+
+class Filter2DFactory : public ISyncTaskFactory {
+ public:
+  InferResult infer(std::span<const TDesc> in, const Json &j) const override {
+    const char *who = "Filter2DFactory";
+    const long long ri = j.at("r_inner").integer(), ro = j.at("r_outer").integer(), si = j.at("s_inner").integer(),
+                    so = j.at("s_outer").integer();
+    check(in.size() == 1, who, "expected exactly one input");
+    const TDesc &d = in[0];
+    check(d.rank() >= 2, who, "input must be a tensor of rank 2 or higher");
+    check(d.dtype == DType::CF32 || d.dtype == DType::F32, who, "input must be complex float32 or real float32");
+    require_device(d, who);
+    check(d.is_contiguous(), who, "input must be C-contiguous");
+    check(ri >= 0, who, "r_inner must be non-negative");
+    check(ro >= 0, who, "r_outer must be non-negative");
+    check(si >= 0, who, "s_inner must be non-negative");
+    check(so >= 0, who, "s_outer must be non-negative");
+    std::vector<InPlace> ip;
+    if (d.dtype == DType::CF32) ip.push_back({0, 0});
+    return sync_result({d}, {d}, ip);
+  }
+  std::unique_ptr<ISyncTask> create(std::span<const TDesc> in, const Json &j, const SyncCreateCtx &ctx) const override {
+    (void)infer(in, j);
+    auto t = std::make_unique<Filter2DTask>();
+    t->st = ctx.stream;
+    t->settings = j;
+    t->idesc = in[0];
+    const auto g = stage_geo(in[0]);
+    HFX_CHECK(hfx_filter2d_create(g.batch, g.H, g.W, in[0].dtype == DType::F32, (int32_t)j.at("r_inner").integer(),
+                                  (int32_t)j.at("r_outer").integer(), (int32_t)j.at("s_inner").integer(),
+                                  (int32_t)j.at("s_outer").integer(), hs(ctx.stream), &t->plan));
+    return t;
+  }
+  // Same settings and input layout keep the task (filter2d.cu:471-497).
+  std::unique_ptr<ISyncTask> update(std::unique_ptr<ISyncTask> old, std::span<const TDesc> in, const Json &j,
+                                    const SyncCreateCtx &ctx) const override {
+    (void)infer(in, j);
+    auto *t = dynamic_cast<Filter2DTask *>(old.get());
+    if (t && in[0] == t->idesc && j == t->settings) {
+      t->st = ctx.stream;
+      return old;
+    }
+    return create(in, j, ctx);
+  }
+};
+
+class FlatfieldTask : public ISyncTask {
+ public:
+  hipStream_t st;
+  Json settings;
+  TDesc idesc;
+  hfx_flatfield *plan = nullptr;
+  ~FlatfieldTask() override {
+    if (plan) hfx_flatfield_destroy(plan);
+  }
+  OpResult execute(SyncCtx &c) override {
+    HFX_CHECK(hfx_flatfield_execute(plan, reinterpret_cast<const float *>(c.inputs[0].data()),
+                                    reinterpret_cast<float *>(c.outputs[0].data()), hs(st)));
+    return OpResult::Ok;
+  }
+};
+
+class FlatfieldFactory : public ISyncTaskFactory {
+ public:
+  InferResult infer(std::span<const TDesc> in, const Json &j) const override {
+    const char *who = "FlatfieldFactory";
+    const float sy = (float)j.at("sigma_y").number(), sx = (float)j.at("sigma_x").number();
+    check(in.size() == 1, who, "expected exactly one input");
+    const TDesc &d = in[0];
+    check(d.rank() >= 2, who, "input must be a tensor of rank 2 or higher");
+    check(d.dtype == DType::F32, who, "input must be float32");
+    require_device(d, who);
+    check(d.is_contiguous(), who, "input must be C-contiguous");
+    check(d.num_elements() > 0, who, "input must not be empty");
+    check(sy > 0.0f, who, "sigma_y must be positive");
+    check(sx > 0.0f, who, "sigma_x must be positive");
+    return sync_result({d}, {d});
+  }
+  std::unique_ptr<ISyncTask> create(std::span<const TDesc> in, const Json &j, const SyncCreateCtx &ctx) const override {
+    (void)infer(in, j);
+    auto t = std::make_unique<FlatfieldTask>();
+    t->st = ctx.stream;
+    t->settings = j;
+    t->idesc = in[0];
+    const auto g = stage_geo(in[0]);
+    const hfx_status rc = hfx_flatfield_create(g.batch, g.H, g.W, (float)j.at("sigma_y").number(),
+                                               (float)j.at("sigma_x").number(), hs(ctx.stream), &t->plan, nullptr);
+    if (rc == HFX_E_INVALID) fail("FlatfieldFactory", hfx_last_error());
+    HFX_CHECK(rc);
+    return t;
+  }
+  // Same settings and input layout keep the task (flatfield.cu:425-451).
+  std::unique_ptr<ISyncTask> update(std::unique_ptr<ISyncTask> old, std::span<const TDesc> in, const Json &j,
+                                    const SyncCreateCtx &ctx) const override {
+    (void)infer(in, j);
+    auto *t = dynamic_cast<FlatfieldTask *>(old.get());
+    if (t && in[0] == t->idesc && j == t->settings) {
       t->st = ctx.stream;
       return old;
     }
@@ -1654,6 +1787,8 @@ void register_hip_nodes(Registry &r) {
   r.register_sync("FFT", std::make_unique<DftFactory<false>>());
   r.register_sync("FresnelDiffraction", std::make_unique<FresnelDiffractionFactory>());
   r.register_sync("AngularSpectrum", std::make_unique<AngularSpectrumFactory>());
+  r.register_sync("Filter2D", std::make_unique<Filter2DFactory>());
+  r.register_sync("Flatfield", std::make_unique<FlatfieldFactory>());
   r.register_sync("MeanAbs", std::make_unique<ReduceFactory<true>>());
   r.register_sync("Mean", std::make_unique<ReduceFactory<false>>());
   r.register_sync("Abs", std::make_unique<AbsFactory>());

@@ -163,6 +163,8 @@ __global__ __launch_bounds__(64 * WROW_WAVES) void k_wrow(PassArgs a) {
         const long long o = p * plane_elems + (long long)y * a.W + x;
         if (a.out_mode == OUT_NAT_MAG) {
           reinterpret_cast<float *>(a.out)[o] = sqrtf(t.x * t.x + t.y * t.y);
+        } else if (a.out_mode == OUT_NAT_RE) {
+          reinterpret_cast<float *>(a.out)[o] = t.x;
         } else {
           if (a.post_lx) t = cmul(t, cmul(a.post_lx[x], a.post_ly[y]));
           reinterpret_cast<float2 *>(a.out)[o] = t;
@@ -281,6 +283,8 @@ __global__ __launch_bounds__(is_mom(KIND) ? 256 * wcol_subsets(N) : 256) void k_
         const long long o = p * plane_elems + (long long)y * a.W + x;
         if (a.out_mode == OUT_NAT_MAG) {
           reinterpret_cast<float *>(a.out)[o] = sqrtf(t.x * t.x + t.y * t.y);
+        } else if (a.out_mode == OUT_NAT_RE) {
+          reinterpret_cast<float *>(a.out)[o] = t.x;
         } else {
           if (a.post_lx) t = cmul(t, cmul(a.post_lx[x], a.post_ly[y]));
           reinterpret_cast<float2 *>(a.out)[o] = t;
@@ -333,7 +337,7 @@ int launch_wave_row_pass(const PassArgs &a, hipStream_t st) {
   }
   if (a.out_mode == OUT_MOMENTS)
     return a.mom.split > 0 ? wrow_dispatch<BLK_TO_MOM5>(a, st) : wrow_dispatch<BLK_TO_MOM>(a, st);
-  if (a.out_mode == OUT_NAT_C || a.out_mode == OUT_NAT_MAG) return wrow_dispatch<BLK_TO_NAT>(a, st);
+  if (a.out_mode == OUT_NAT_C || a.out_mode == OUT_NAT_MAG || a.out_mode == OUT_NAT_RE) return wrow_dispatch<BLK_TO_NAT>(a, st);
   return -4;
 }
 
@@ -345,7 +349,7 @@ int launch_wave_col_pass(const PassArgs &a, hipStream_t st) {
   }
   if (a.out_mode == OUT_MOMENTS)
     return a.mom.split > 0 ? wcol_dispatch<BLK_TO_MOM5>(a, st) : wcol_dispatch<BLK_TO_MOM>(a, st);
-  if (a.out_mode == OUT_NAT_C || a.out_mode == OUT_NAT_MAG) return wcol_dispatch<BLK_TO_NAT>(a, st);
+  if (a.out_mode == OUT_NAT_C || a.out_mode == OUT_NAT_MAG || a.out_mode == OUT_NAT_RE) return wcol_dispatch<BLK_TO_NAT>(a, st);
   return -4;
 }
 

@@ -158,6 +158,31 @@ hfx_status hfx_asm_create(const hfx_asm_desc *d, hfx_stream_t stream, hfx_asm **
 hfx_status hfx_asm_execute(hfx_asm *a, const void *in, void *out, hfx_stream_t stream);
 hfx_status hfx_asm_destroy(hfx_asm *a);
 
+/* holotask::syncs::Filter2D (filter2d.cu:165-497): out = IFFT2_unnorm(FFT2(in) * F) over
+ * [batch][H][W], F the cosine-tapered annulus a(1-b) of radii r_inner/r_outer and tapers
+ * s_inner/s_outer.  CF32 input (in_real = 0): F is the reference's centred table after the
+ * corner exchange, a(1-b) in both real and imaginary parts, and `out` may alias `in`.
+ * F32 input (in_real = 1): F is real on the DC-centred frequency grid and `out` is F32 (the
+ * real-to-complex / complex-to-real pair).  Sizes: H and W in 2^a*3^b, 2..2048. */
+typedef struct hfx_filter2d hfx_filter2d;
+hfx_status hfx_filter2d_create(int64_t batch, int32_t height, int32_t width, int32_t in_real, int32_t r_inner,
+                               int32_t r_outer, int32_t s_inner, int32_t s_outer, hfx_stream_t stream,
+                               hfx_filter2d **out);
+hfx_status hfx_filter2d_execute(hfx_filter2d *f, const void *in, void *out, hfx_stream_t stream);
+hfx_status hfx_filter2d_destroy(hfx_filter2d *f);
+
+/* holotask::syncs::Flatfield (flatfield.cu:73-451): out = in - GaussianBlur(in) per
+ * [H][W] plane of F32 [batch][H][W].  Kernels of radius ceil(3 sigma) (sizes below 65, or a
+ * one-pixel-wide or -tall plane): separable blur with clamp-at-edge indexing; larger
+ * kernels: the Fourier-domain high-pass (1 - exp(-2 pi^2 (sx^2 fx^2 + sy^2 fy^2))) * scale,
+ * with sigma and scale = 1/(H*W) rounded to six decimals as the reference's generated
+ * source spells them.  `out` must not alias `in`.  *uses_fft reports the form chosen. */
+typedef struct hfx_flatfield hfx_flatfield;
+hfx_status hfx_flatfield_create(int64_t batch, int32_t height, int32_t width, float sigma_y, float sigma_x,
+                                hfx_stream_t stream, hfx_flatfield **out, int32_t *uses_fft);
+hfx_status hfx_flatfield_execute(hfx_flatfield *f, const float *in, float *out, hfx_stream_t stream);
+hfx_status hfx_flatfield_destroy(hfx_flatfield *f);
+
 /* holotask::syncs::MeanAbs (mean_abs.cu:231-276) / holonp::Mean (mean.cu:254-306) /
  * Multiply+Mean (multiply.cu:69-92) over the plane axis of [planes][npix]:
  *   out[p] = (1/planes) * sum_k w_k * m_k(p),  m = |x| for complex input, and x (or |x|

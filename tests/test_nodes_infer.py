@@ -305,3 +305,45 @@ def test_dual_reader_batch_queue_infer():
     assert r["owned_outputs"] == [True, True, False]
     with pytest.raises(HfxInvalidArgument):
         nodes.infer("DualReaderBatchQueue", {"target_capacity": 0, "window_size": 5}, [D([8, 3], "F32")])
+
+
+# ---------------------------------------------------------------- Filter2D / Flatfield (filter2d_test.cu, flatfield_test.cu)
+F2D = {"r_inner": 0, "r_outer": 100, "s_inner": 0, "s_outer": 0}
+
+
+def test_filter2d_infer_real_and_complex():
+    r = nodes.infer("Filter2D", F2D, [D([2, 2], "F32")])
+    assert r["kind"] == "Sync" and r["in_place"] == []
+    assert r["output_descs"][0]["shape"] == [2, 2] and r["output_descs"][0]["dtype"] == "F32"
+    r = nodes.infer("Filter2D", F2D, [D([3, 2, 2], "CF32")])
+    assert r["in_place"] == [[0, 0]] and r["output_descs"][0]["dtype"] == "CF32"
+
+
+@pytest.mark.parametrize("inp,st,msg", [
+    (D([4], "F32"), F2D, "rank 2 or higher"),
+    (D([4, 4], "U16"), F2D, "complex float32 or real float32"),
+    (D([4, 4], "F32", mem_loc="Host"), F2D, "device memory"),
+    (D([4, 4], "F32", strides=[4, 16]), F2D, "C-contiguous"),
+    (D([4, 4], "F32"), dict(F2D, r_inner=-1), "r_inner must be non-negative"),
+    (D([4, 4], "F32"), dict(F2D, s_outer=-2), "s_outer must be non-negative"),
+])
+def test_filter2d_rejections(inp, st, msg):
+    with pytest.raises(HfxInvalidArgument, match=msg):
+        nodes.infer("Filter2D", st, [inp])
+
+
+def test_flatfield_infer():
+    r = nodes.infer("Flatfield", {"sigma_y": 1.0, "sigma_x": 1.0}, [D([2, 3, 4], "F32")])
+    assert r["kind"] == "Sync" and r["in_place"] == []
+    assert r["output_descs"][0]["shape"] == [2, 3, 4] and r["output_descs"][0]["dtype"] == "F32"
+
+
+@pytest.mark.parametrize("inp,st,msg", [
+    (D([2, 2], "F32"), {"sigma_y": 0.0, "sigma_x": 1.0}, "sigma_y must be positive"),
+    (D([2, 2], "F32"), {"sigma_y": 1.0, "sigma_x": 0.0}, "sigma_x must be positive"),
+    (D([2, 2], "CF32"), {"sigma_y": 1.0, "sigma_x": 1.0}, "input must be float32"),
+    (D([0, 2], "F32"), {"sigma_y": 1.0, "sigma_x": 1.0}, "must not be empty"),
+])
+def test_flatfield_rejections(inp, st, msg):
+    with pytest.raises(HfxInvalidArgument, match=msg):
+        nodes.infer("Flatfield", st, [inp])
