@@ -143,6 +143,66 @@ def serving_leg(args, np, torch, load=0.7, slots=16):
     return summary
 
 
+def stages_leg(args, np, torch, ring, n_windows, cfg, value):
+    """Optional in-path stages at the workload's sizes: Filter2D on the 15 band planes of a
+    window (CF32 [15, 1024, 1024]), Flatfield on the three moment images (spatial, sigma 3,
+    and the Fourier form, sigma 40), each timed alone with HIP events; then the fused path
+    with both enabled."""
+    import ctypes as C
+
+    from holoflow_amd import hfx
+    from holoflow_amd._lib import check, lib
+
+    stream = torch.cuda.current_stream()
+    sp = C.c_void_p(stream.cuda_stream)
+
+    def timed(fn, loops):
+        for _ in range(3):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(loops):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / loops
+
+    res = {}
+    planes = torch.randn(15, H, W, dtype=torch.complex64, device="cuda")
+    f = C.c_void_p()
+    check(lib().hfx_filter2d_create(15, H, W, 0, 3, 200, 2, 10, sp, C.byref(f)))
+    ms = timed(lambda: check(lib().hfx_filter2d_execute(f, C.c_void_p(planes.data_ptr()),
+                                                        C.c_void_p(planes.data_ptr()), sp)), 50)
+    lib().hfx_filter2d_destroy(f)
+    nbytes = 2 * planes.numel() * 8
+    res["filter2d_15x1024x1024_cf32"] = {"us": round(ms * 1e3, 2), "algorithmic_bytes": nbytes,
+                                          "gbs": round(nbytes / (ms * 1e-3) / 1e9, 1), "passes": 3}
+    img = torch.rand(3, H, W, device="cuda")
+    dst = torch.empty_like(img)
+    for name, sigma in (("flatfield_3x1024x1024_spatial_sigma3", 3.0), ("flatfield_3x1024x1024_fourier_sigma40", 40.0)):
+        h = C.c_void_p()
+        check(lib().hfx_flatfield_create(3, H, W, sigma, sigma, sp, C.byref(h), None))
+        ms = timed(lambda: check(lib().hfx_flatfield_execute(h, C.c_void_p(img.data_ptr()),
+                                                             C.c_void_p(dst.data_ptr()), sp)), 100)
+        lib().hfx_flatfield_destroy(h)
+        nbytes = 2 * img.numel() * 4
+        res[name] = {"us": round(ms * 1e3, 2), "algorithmic_bytes": nbytes,
+                     "gbs": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+    del planes, img, dst
+    c = cfg("fp32")
+    c.filt = (3, 200, 2, 10)
+    c.flatfield = (3.0, 3.0)
+    plan = hfx.MomentPlan(c)
+    out = plan.new_output(n_windows)
+    dt = time_batch(plan, ring, n_windows, out, max(1, args.steps // 2), 1, torch)
+    fps = n_windows * max(1, args.steps // 2) / dt
+    res["path_with_filter2d_and_flatfield"] = {"value": round(fps, 2), "unit": "moment frames/s",
+                                               "fraction_of_plain_path": round(fps / value, 4)}
+    plan.close()
+    del out
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +214,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-cfg4", action="store_true")
+    ap.add_argument("--no-stages", action="store_true")
     ap.add_argument("--cfg4-frames", type=int, default=8192)
     ap.add_argument("--serving-streams", type=int, default=24)
     ap.add_argument("--serving-windows", type=int, default=128)
@@ -280,6 +341,11 @@ def main():
         mixed["fp32_rel_l2_vs_fp64"] = {k: float(f"{fp32_err[i]:.3e}") for i, k in enumerate(("M0", "M1", "M2"))}
 
     del out
+    # ---- optional in-path stages (Filter2D, Flatfield) at the workload's sizes
+    stages = None
+    if not args.no_stages:
+        stages = stages_leg(args, np, torch, ring, n_windows, cfg, value)
+        log(f"stages: {stages}")
     # ---- large single stream (2048^2, T=128, band split)
     big = None
     if not args.no_cfg4:
@@ -343,6 +409,7 @@ def main():
                        "window_gpu_p99": round(float(np.percentile(lat, 99)), 4),
                        "window_period": round(window_period_ms, 4)},
         "mixed": mixed,
+        "stages": stages,
         "cfg4": big,
         "serving": serve,
         "cpu_baseline": cpu,

@@ -69,6 +69,8 @@ class PlanDesc(C.Structure):
         ("r_inner", C.c_double), ("r_outer", C.c_double), ("s_inner", C.c_double), ("s_outer", C.c_double),
         ("split_bin", C.c_int32),
         ("weight_fs_hz", C.c_double),
+        ("flatfield_enabled", C.c_int32),
+        ("flatfield_sigma_y", C.c_float), ("flatfield_sigma_x", C.c_float),
     ]
 
 

@@ -87,6 +87,16 @@ void build_weights(int kb, int ke, int T, double fs_w, bool both, std::vector<fl
 
 }  // namespace
 
+// A spectral filter: forward row pass -> column pass with the table between its forward
+// and inverse transforms -> inverse row pass (Filter2D, and Flatfield's Fourier form).
+struct SpectralFilter {
+  int64_t batch = 0;
+  int H = 0, W = 0, blk_c = 1;
+  DevArena arena;
+  float2 *tw_w = nullptr, *tw_h = nullptr, *tf = nullptr;
+  float2 *ws = nullptr;  // [batch][H][W] blocked spectrum: owned, or borrowed from a plan
+};
+
 struct hfx_plan {
   hfx_plan_desc d{};
   int H = 0, W = 0, OH = 0, OW = 0;
@@ -102,6 +112,11 @@ struct hfx_plan {
   float2 *lx = nullptr, *ly = nullptr;
   float2 *tf = nullptr;
   float *wf = nullptr, *wf2 = nullptr;
+  bool filter2d = false;     // Fresnel: Filter2D on the band planes before the lens
+  SpectralFilter f2d;        // its tables; workspace = planes_blk
+  hfx_flatfield *ff = nullptr;  // Flatfield on the moment images
+  float *mom_tmp = nullptr;  // moment images before the flatfield
+  ~hfx_plan();
 };
 
 struct hfx_fresnel {
@@ -120,15 +135,6 @@ struct hfx_asm {
   float2 *tw_w = nullptr, *tw_h = nullptr, *tf = nullptr, *ws = nullptr;
 };
 
-// A spectral filter: forward row pass -> column pass with the table between its forward
-// and inverse transforms -> inverse row pass (Filter2D, and Flatfield's Fourier form).
-struct SpectralFilter {
-  int64_t batch = 0;
-  int H = 0, W = 0, blk_c = 1;
-  DevArena arena;
-  float2 *tw_w = nullptr, *tw_h = nullptr, *tf = nullptr, *ws = nullptr;
-};
-
 struct hfx_filter2d {
   SpectralFilter sf;
   int in_real = 0;
@@ -143,6 +149,16 @@ struct hfx_flatfield {
   float *kx = nullptr, *ky = nullptr, *tmp = nullptr;
   SpectralFilter sf;
 };
+
+hfx_plan::~hfx_plan() {
+  if (ff) (void)hfx_flatfield_destroy(ff);
+}
+
+static hfx_status spectral_setup(SpectralFilter &f, int64_t batch, int H, int W, hipStream_t st,
+                                 float2 *borrowed_ws = nullptr);
+static hfx_status spectral_run(const SpectralFilter &f, const void *in, bool in_real, void *out, bool out_real,
+                               hipStream_t st);
+static hfx_status window_back_moments(hfx_plan *p, float *out, hipStream_t st);
 
 extern "C" {
 
@@ -205,8 +221,17 @@ hfx_status hfx_plan_create(const hfx_plan_desc *dp, hfx_stream_t stream, hfx_pla
   if (d.prop != HFX_PROP_FRESNEL && d.prop != HFX_PROP_ANGULAR_SPECTRUM)
     return fail(HFX_E_INVALID, "unknown propagation");
   if (d.prop == HFX_PROP_FRESNEL && !(d.z_m != 0)) return fail(HFX_E_INVALID, "Fresnel z must be non-zero");
-  if (d.prop == HFX_PROP_FRESNEL && (d.pad_height || d.pad_width || d.filter_enabled))
-    return fail(HFX_E_INVALID, "padding and filter apply to the angular spectrum only");
+  if (d.prop == HFX_PROP_FRESNEL && (d.pad_height || d.pad_width))
+    return fail(HFX_E_INVALID, "padding applies to the angular spectrum only");
+  if (d.filter_enabled && !(d.r_inner >= 0 && d.r_outer >= 0 && d.s_inner >= 0 && d.s_outer >= 0))
+    return fail(HFX_E_INVALID, "filter radii and tapers must be non-negative");
+  if (d.filter_enabled && d.prop == HFX_PROP_FRESNEL &&
+      (d.r_inner != std::floor(d.r_inner) || d.r_outer != std::floor(d.r_outer) ||
+       d.s_inner != std::floor(d.s_inner) || d.s_outer != std::floor(d.s_outer) || d.r_outer > 1e9 ||
+       d.r_inner > 1e9 || d.s_inner > 1e9 || d.s_outer > 1e9))
+    return fail(HFX_E_INVALID, "Filter2D radii and tapers must be integers");
+  if (d.flatfield_enabled && !(d.flatfield_sigma_y > 0.0f && d.flatfield_sigma_x > 0.0f))
+    return fail(HFX_E_INVALID, "flatfield sigma must be positive");
   const int nb = d.bin_end - d.bin_begin;
   if (d.split_bin != 0 && (d.split_bin <= d.bin_begin || d.split_bin >= d.bin_end || both))
     return fail(HFX_E_INVALID, "split_bin must lie strictly inside the one-sided band");
@@ -236,7 +261,8 @@ hfx_status hfx_plan_create(const hfx_plan_desc *dp, hfx_stream_t stream, hfx_pla
 
   hipStream_t st = S(stream);
   DevArena &A = p->arena;
-  p->fused_row = p->OH == p->H && p->OW == p->W && hfx::dft_row_supported(p->W, p->nbc, d.window);
+  p->filter2d = d.prop == HFX_PROP_FRESNEL && d.filter_enabled;
+  p->fused_row = !p->filter2d && p->OH == p->H && p->OW == p->W && hfx::dft_row_supported(p->W, p->nbc, d.window);
   // natural-layout planes: only the stand-alone time DFT needs them
   HFX_TRY(A.alloc(&p->planes_nat, p->fused_row ? 1 : (size_t)p->n_planes * p->npix));
   HFX_TRY(A.alloc(&p->planes_blk, (size_t)p->n_planes * p->onpix));
@@ -260,6 +286,16 @@ hfx_status hfx_plan_create(const hfx_plan_desc *dp, hfx_stream_t stream, hfx_pla
   hfx::launch_dft_matrix(p->wm32, p->wm16hi, p->wm16lo, p->wmbf, p->wm_rows, d.window,
                          d.bin_begin, p->nbc, st);
   HFX_TRY(gen_prop_tables(p, st));
+  if (p->filter2d) {
+    HFX_TRY(spectral_setup(p->f2d, p->n_planes, p->H, p->W, st, p->planes_blk));
+    hfx::launch_filter2d_table(p->f2d.tf, p->H, p->W, (int)d.r_inner, (int)d.r_outer, (int)d.s_inner,
+                               (int)d.s_outer, 1, p->f2d.blk_c, st);
+  }
+  if (d.flatfield_enabled) {
+    HFX_TRY(A.alloc(&p->mom_tmp, (size_t)p->n_out * p->onpix));
+    HFX_TRY(hfx_flatfield_create(p->n_out, p->OH, p->OW, d.flatfield_sigma_y, d.flatfield_sigma_x, stream, &p->ff,
+                                 nullptr));
+  }
   std::vector<float> f, f2;
   double fs_w = d.weight_fs_hz > 0 ? d.weight_fs_hz : d.fs_hz;
   build_weights(d.bin_begin, d.bin_end, d.window, fs_w, both, f, f2);
@@ -281,7 +317,7 @@ hfx_status hfx_plan_get_info(const hfx_plan *p, hfx_plan_info *info) {
   info->out_width = p->OW;
   info->n_outputs = p->n_out;
   info->n_planes = p->n_planes;
-  info->workspace_bytes = (int64_t)p->arena.bytes;
+  info->workspace_bytes = (int64_t)(p->arena.bytes + p->f2d.arena.bytes);
   return HFX_OK;
 }
 
@@ -316,8 +352,9 @@ static hfx::DftLaunch dft_launch(hfx_plan *p, const uint16_t *ring, int64_t ring
   L.nbc = p->nbc;
   L.both = (p->d.flags & HFX_FLAG_BOTH_SIDEBANDS) ? 1 : 0;
   L.planes = planes;
-  L.lx = p->d.prop == HFX_PROP_FRESNEL ? p->lx : nullptr;
-  L.ly = p->d.prop == HFX_PROP_FRESNEL ? p->ly : nullptr;
+  // the Fresnel lens multiplies the DFT output, unless Filter2D comes between them
+  L.lx = p->d.prop == HFX_PROP_FRESNEL && !p->filter2d ? p->lx : nullptr;
+  L.ly = p->d.prop == HFX_PROP_FRESNEL && !p->filter2d ? p->ly : nullptr;
   L.precision = p->d.precision;
   L.wm_f32 = p->wm32;
   L.wm_f16hi = reinterpret_cast<const uint16_t *>(p->wm16hi);
@@ -365,8 +402,15 @@ static hfx_status window_front(hfx_plan *p, const uint16_t *ring, int64_t ring_f
     return HFX_OK;
   }
   HFX_TRY(run_dft(p, ring, ring_frames, first, p->planes_nat, st));
-  // row pass: natural planes (lens already applied for Fresnel) -> blocked spectrum rows
+  // Filter2D in place on the natural planes, through the blocked workspace
+  if (p->filter2d) HFX_TRY(spectral_run(p->f2d, p->planes_nat, false, p->planes_nat, false, st));
+  // row pass: natural planes (lens already applied for Fresnel, or here after Filter2D)
+  // -> blocked spectrum rows
   hfx::PassArgs r = base_pass(p->OH, p->OW, p->n_planes, p->blk_c);
+  if (p->filter2d) {
+    r.pre_lx = p->lx;
+    r.pre_ly = p->ly;
+  }
   r.in = p->planes_nat;
   r.in_mode = hfx::IN_NAT_C;
   r.in_h = p->H;
@@ -383,6 +427,12 @@ static hfx_status window_front(hfx_plan *p, const uint16_t *ring, int64_t ring_f
 // Back of the schedule: column pass (Fresnel: with |.| and the moment reduction; ASM: with
 // the transfer function, then the inverse row pass with the reduction) into `out`.
 static hfx_status window_back(hfx_plan *p, float *out, hipStream_t st) {
+  HFX_TRY(window_back_moments(p, p->ff ? p->mom_tmp : out, st));
+  if (p->ff) return hfx_flatfield_execute(p->ff, p->mom_tmp, out, reinterpret_cast<hfx_stream_t>(st));
+  return HFX_OK;
+}
+
+static hfx_status window_back_moments(hfx_plan *p, float *out, hipStream_t st) {
   const long long on = p->onpix;
   hfx::MomentArgs mom{};
   for (int i = 0; i < p->n_out; ++i) mom.out[i] = out + i * on;
@@ -634,7 +684,8 @@ hfx_status hfx_asm_destroy(hfx_asm *a) {
 
 }  // extern "C"
 
-static hfx_status spectral_setup(SpectralFilter &f, int64_t batch, int H, int W, hipStream_t st) {
+static hfx_status spectral_setup(SpectralFilter &f, int64_t batch, int H, int W, hipStream_t st,
+                                 float2 *borrowed_ws) {
   if (batch <= 0 || H <= 0 || W <= 0) return fail(HFX_E_INVALID, "invalid shape");
   if (batch > 65535) return fail(HFX_E_UNSUPPORTED, "more than 65535 planes");
   f.batch = batch;
@@ -646,7 +697,10 @@ static hfx_status spectral_setup(SpectralFilter &f, int64_t batch, int H, int W,
   HFX_TRY(f.arena.alloc(&f.tw_w, W));
   HFX_TRY(f.arena.alloc(&f.tw_h, H));
   HFX_TRY(f.arena.alloc(&f.tf, (size_t)H * W));
-  HFX_TRY(f.arena.alloc(&f.ws, (size_t)batch * H * W));
+  if (borrowed_ws)
+    f.ws = borrowed_ws;
+  else
+    HFX_TRY(f.arena.alloc(&f.ws, (size_t)batch * H * W));
   hfx::launch_fft_twiddles(f.tw_w, W, st);
   hfx::launch_fft_twiddles(f.tw_h, H, st);
   return HFX_OK;

@@ -1159,6 +1159,29 @@ hfx_plan_desc moments_desc(const Json &j, const TDesc &d) {
     p.s_outer = (double)f.at("s_outer").integer();
   }
   p.split_bin = (int32_t)j.get_int("split_bin", 0);
+  if (j.contains("flatfield") && !j.at("flatfield").is_null()) {
+    const auto &ff = j.at("flatfield");
+    p.flatfield_enabled = 1;
+    if (ff.contains("cutoff_period_m")) {
+      // sigma = 0.187 * period / image pitch, the Fresnel image pitch being lambda |z| / (N d)
+      // (graph_builder.cc:32-78), in FP32 as the reference's settings hold them
+      const float period = (float)ff.at("cutoff_period_m").number();
+      float pitch_y = (float)p.dy_m, pitch_x = (float)p.dx_m;
+      if (p.prop == HFX_PROP_FRESNEL) {
+        const float lam = (float)p.lambda_m, z = std::fabs((float)p.z_m);
+        if (!(z > 0.0f)) fail("DopplerMomentsFactory", "Fresnel output pitch requires non-zero propagation distance");
+        pitch_y = lam * z / ((float)p.height * (float)p.dy_m);
+        pitch_x = lam * z / ((float)p.width * (float)p.dx_m);
+      }
+      if (!(period > 0.0f && pitch_y > 0.0f && pitch_x > 0.0f))
+        fail("DopplerMomentsFactory", "flatfield cutoff period and image pitches must be positive");
+      p.flatfield_sigma_y = 0.187f * period / pitch_y;
+      p.flatfield_sigma_x = 0.187f * period / pitch_x;
+    } else {
+      p.flatfield_sigma_y = (float)ff.at("sigma_y").number();
+      p.flatfield_sigma_x = (float)ff.at("sigma_x").number();
+    }
+  }
   return p;
 }
 
@@ -1192,6 +1215,8 @@ class DopplerMomentsFactory : public ISyncTaskFactory {
     const auto p = moments_desc(j, d);
     check(p.lambda_m > 0 && p.dx_m > 0 && p.dx_m == p.dy_m, who, "invalid optics (lambda > 0, dx == dy > 0)");
     check(p.bin_begin >= 0 && p.bin_end > p.bin_begin, who, "band must satisfy 0 <= begin < end");
+    check(!p.flatfield_enabled || (p.flatfield_sigma_y > 0.0f && p.flatfield_sigma_x > 0.0f), who,
+          "flatfield sigma must be positive");
     const size_t n_out = p.split_bin > 0 ? 5 : 3;
     const size_t oh = p.pad_height > 0 ? (size_t)p.pad_height : (size_t)p.height;
     const size_t ow = p.pad_width > 0 ? (size_t)p.pad_width : (size_t)p.width;
@@ -1218,7 +1243,11 @@ class DopplerMomentsFactory : public ISyncTaskFactory {
     Json a = j, b = t->settings;
     a["z"] = Json(0.0);
     b["z"] = Json(0.0);
-    if (a == b) {
+    // a flatfield set by its cutoff period follows the Fresnel image pitch, which moves with z
+    const bool pitch_follows_z = j.contains("flatfield") && !j.at("flatfield").is_null() &&
+                                 j.at("flatfield").contains("cutoff_period_m") &&
+                                 moments_desc(j, in[0]).prop == HFX_PROP_FRESNEL;
+    if (a == b && !pitch_follows_z) {
       (void)infer(in, j);
       t->st = ctx.stream;
       HFX_CHECK(hfx_plan_update_z(t->plan, f32(j, "z"), hs(ctx.stream)));

@@ -50,8 +50,9 @@ class MomentConfig:
     power2: bool = False
     both_sidebands: bool = False
     pad: tuple = (0, 0)
-    filt: Optional[tuple] = None
+    filt: Optional[tuple] = None           # (r_inner, r_outer, s_inner, s_outer)
     split_bin: int = 0
+    flatfield: Optional[tuple] = None      # (sigma_y, sigma_x)
 
     def to_desc(self) -> _lib.PlanDesc:
         d = _lib.PlanDesc()
@@ -72,6 +73,9 @@ class MomentConfig:
             d.r_inner, d.r_outer, d.s_inner, d.s_outer = (float(v) for v in self.filt)
         d.split_bin = self.split_bin
         d.weight_fs_hz = self.weight_fs_hz
+        if self.flatfield is not None:
+            d.flatfield_enabled = 1
+            d.flatfield_sigma_y, d.flatfield_sigma_x = (float(v) for v in self.flatfield)
         return d
 
 

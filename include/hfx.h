@@ -69,11 +69,16 @@ typedef struct hfx_plan_desc {
   int32_t precision;        /* enum hfx_precision */
   uint32_t flags;           /* enum hfx_flags */
   int32_t pad_height, pad_width; /* angular spectrum centred zero padding, 0 = none */
-  int32_t filter_enabled;   /* angular spectrum annular filter */
+  int32_t filter_enabled;   /* annular filter (graph_builder.cc:124-126, 546-553): the angular
+                               spectrum's transfer-function filter, or for Fresnel a Filter2D
+                               stage on the band planes before propagation (integer radii) */
   double r_inner, r_outer, s_inner, s_outer;
   int32_t split_bin;        /* >0: also M0 over [begin, split) and [split, end) */
   double weight_fs_hz;      /* sampling rate of the M1/M2 weights f_k = k*fs/T; the
                                reference hard-codes 37000 (graph_builder.cc:580) */
+  int32_t flatfield_enabled; /* Flatfield on every moment image after the fftshift
+                                (graph_builder.cc:637-641) */
+  float flatfield_sigma_y, flatfield_sigma_x;
 } hfx_plan_desc;
 
 typedef struct hfx_plan_info {

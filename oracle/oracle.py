@@ -26,6 +26,10 @@ The same code runs in two precisions:
 All propagation phases (Fresnel lens, angular-spectrum transfer function) are generated
 in float64 in both precisions and rounded once to complex64 for "f32".
 
+Optional stages (oracle/stages.py): Filter2D on the band planes before Fresnel
+(graph_builder.cc:124-126) and Flatfield on each moment image after the fftshift
+(graph_builder.cc:637-641), when PathParams.filt (Fresnel) / PathParams.flatfield are set.
+
 Pinning (tests/test_oracle.py): reference KATs (fft_test.cu:86, rfft_test.cu:86,
 fftshift_test.cu:105-121, mean_test.cu:109-141, angular_spectrum_test.cu:146-283),
 brute-force O(n^2) DFTs, the discrete Gauss-sum identity for the Fresnel lens, plane-wave
@@ -67,8 +71,9 @@ class PathParams:
     fftshift: bool = False
     power2: bool = False
     pad: tuple = (0, 0)         # (pad_height, pad_width), 0 = none
-    filt: Optional[tuple] = None  # (r_inner, r_outer, s_inner, s_outer)
+    filt: Optional[tuple] = None  # (r_inner, r_outer, s_inner, s_outer): ASM filter, or Filter2D before Fresnel
     split_bin: int = 0
+    flatfield: Optional[tuple] = None  # (sigma_y, sigma_x): Flatfield on every moment image
 
     @property
     def n_bins(self) -> int:
@@ -256,12 +261,19 @@ def doppler_moments(window: np.ndarray, p: PathParams, prec: str = "f64") -> np.
     """Full path for one [T][H][W] uint16 window -> [n_out][OH][OW] moment images."""
     S = time_dft(window, p, prec)
     if p.prop == "fresnel":
+        if p.filt is not None:      # Filter2D on the band planes (graph_builder.cc:124-126)
+            from . import stages
+            S = torch.from_numpy(stages.filter2d(S.numpy(), *(int(v) for v in p.filt))).to(S.dtype)
         P = fresnel(S, p, prec, output_magnitude=True)
     elif p.prop == "asm":
         P = angular_spectrum(S, p, prec)
     else:
         raise ValueError(f"unknown propagation {p.prop!r}")
-    return moments(P, p, prec).numpy()
+    M = moments(P, p, prec).numpy()
+    if p.flatfield is not None:     # after the fftshift (graph_builder.cc:633-641)
+        from . import stages
+        M = stages.flatfield(M, *p.flatfield).astype(M.dtype)
+    return M
 
 
 def rel_l2(x, ref) -> float:

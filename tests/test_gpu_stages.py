@@ -9,9 +9,12 @@ import numpy as np
 import pytest
 import torch
 
+from holoflow_amd import hfx
 from holoflow_amd._lib import check, lib
-from holoflow_amd.nodes import Node, desc_of
+from holoflow_amd.nodes import Node, desc, desc_of
+from oracle import oracle as O
 from oracle import stages as OS
+from oracle.synth import SynthParams, synth_frames
 from tests.conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -99,3 +102,50 @@ def test_stage_update_reuse(gpu):
     x = torch.randn(2, 64, 64, device="cuda")
     np.testing.assert_allclose(g.execute(x)[0].cpu().numpy(), OS.flatfield(x.cpu().numpy(), 2.5, 2.0),
                                atol=1e-5)
+
+
+# ------------------------------------------------------------------ inside the fused plan
+
+
+@pytest.mark.parametrize("prop,filt,ff", [
+    ("fresnel", (3, 20, 2, 5), None),           # Filter2D before the Fresnel lens
+    ("fresnel", None, (2.0, 3.0)),              # spatial flatfield on the moment images
+    ("fresnel", (0, 25, 0, 4), (15.0, 12.0)),   # both; Fourier flatfield
+    ("asm", (10, 25, 3, 3), (2.5, 2.5)),        # angular-spectrum filter + flatfield
+])
+def test_fused_plan_with_optional_stages(gpu, prop, filt, ff):
+    T, H, W, S = 32, 128, 128, 8
+    kb, ke = O.band_bins(6e3, 15e3, 37e3, T)
+    p = O.PathParams(H, W, T, S, kb, ke, prop, z_m=0.3, fftshift=True, filt=filt, flatfield=ff)
+    frames, _ = synth_frames(SynthParams(H, W, 30.0, seed=4, stream_id=3), 0, T)
+    ref = O.doppler_moments(frames, p, "f64")
+    plan = hfx.MomentPlan(hfx.MomentConfig(H, W, T, S, kb, ke, prop, z_m=0.3, fftshift=True, filt=filt,
+                                           flatfield=ff))
+    ring = torch.from_numpy(frames.view(np.int16)).cuda()
+    out = plan.moments(ring, first_frame=0)
+    torch.cuda.synchronize()
+    out = out.cpu().numpy()
+    for m in range(3):
+        assert rel_l2(out[m], ref[m]) < 1e-5, f"moment {m}"
+
+
+def test_moments_node_flatfield_by_cutoff_period(gpu):
+    T, H, W = 32, 128, 128
+    kb, ke = O.band_bins(6e3, 15e3, 37e3, T)
+    lam, dx, z, period = 852e-9, 12e-6, 0.3, 2.67e-3
+    f = np.float32
+    pitch = f(lam) * abs(f(z)) / (f(H) * f(dx))          # Fresnel image pitch, FP32
+    sigma = float(f(0.187) * f(period) / pitch)
+    p = O.PathParams(H, W, T, T, kb, ke, "fresnel", lambda_m=float(f(lam)), dx_m=float(f(dx)), dy_m=float(f(dx)),
+                     z_m=float(f(z)), fftshift=True, flatfield=(sigma, sigma))
+    frames, _ = synth_frames(SynthParams(H, W, 30.0, seed=6, stream_id=1), 0, T)
+    n = Node("DopplerMoments", {"lambda": lam, "dx": dx, "z": z, "time_z_begin": kb, "time_z_end": ke,
+                                "prop": "fresnel", "fftshift": True, "flatfield": {"cutoff_period_m": period}},
+             [desc([T, H, W], "U16")])
+    out = n.execute(torch.from_numpy(frames.view(np.int16)).cuda())[0]
+    torch.cuda.synchronize()
+    ref = O.doppler_moments(frames, p, "f64")
+    assert rel_l2(out.cpu().numpy(), ref) < 1e-5
+    # z moves the Fresnel image pitch and with it the flatfield sigma: the task is rebuilt
+    assert not n.update({"lambda": lam, "dx": dx, "z": 0.35, "time_z_begin": kb, "time_z_end": ke,
+                         "prop": "fresnel", "fftshift": True, "flatfield": {"cutoff_period_m": period}})
