@@ -223,6 +223,21 @@ class HolofileInfo(C.Structure):
 
 
 SIGNATURES.update({
+    # include/hfx_graph.h
+    "hfx_graph_compile": (C.c_int, [C.c_char_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "hfx_graph_destroy": (C.c_int, [C.c_void_p]),
+    "hfx_graph_describe": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
+    "hfx_graph_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    "hfx_graph_start": (C.c_int, [C.c_void_p]),
+    "hfx_graph_stop": (C.c_int, [C.c_void_p]),
+    "hfx_graph_metrics": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
+    "hfx_graph_output": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32, C.POINTER(C.c_void_p), C.c_char_p,
+                                   C.c_int64]),
+    "hfx_graph_dot": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int64]),
+})
+
+
+SIGNATURES.update({
     # include/hfx_holofile.h
     "hfx_holofile_open": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "hfx_holofile_close": (C.c_int, [C.c_void_p]),

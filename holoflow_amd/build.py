@@ -17,7 +17,7 @@ ROOT = os.path.dirname(HERE)
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhfx.so")
 
-SOURCES = ["tables.hip", "dft.hip", "fft2d.hip", "wpass.hip", "ops.hip", "synth.hip", "filters.hip", "api.hip", "nodes.cpp", "serve.cpp", "holofile.cpp"]
+SOURCES = ["tables.hip", "dft.hip", "fft2d.hip", "wpass.hip", "ops.hip", "synth.hip", "filters.hip", "api.hip", "nodes.cpp", "serve.cpp", "holofile.cpp", "graph_spec.cpp", "runtime.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -54,7 +54,7 @@ def build(verbose: bool = True) -> str:
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [_hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
+    cmd = [_hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-pthread", "-o", tmp]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr}")

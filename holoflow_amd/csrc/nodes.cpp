@@ -17,6 +17,7 @@
 //   FFTShiftNp                  fftshift.cu:110-260
 //   DopplerMoments (fused)      the whole chain graph_builder.cc:245-305, 522-566, 604-647
 //   Holofile (source)           sources/holofile.cc:244-434
+//   Memcpy                      memcpy.cc:105-196
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -1803,6 +1804,65 @@ class HolofileFactory : public ISyncTaskFactory {
   }
 };
 
+// ================================================================== Memcpy
+// Host/device transfer (memcpy.cc:105-196): the output is the input's descriptor in the
+// target memory; device transfers are enqueued on the node's stream, host to host copies
+// run on the calling thread.
+class MemcpyTask : public ISyncTask {
+ public:
+  hipStream_t st;
+  MemLoc target = MemLoc::Device;
+  OpResult execute(SyncCtx &c) override {
+    const TView &in = c.inputs[0];
+    const size_t n = c.outputs[0].desc.num_bytes();
+    const MemLoc src = in.desc.mem_loc;
+    if (src == MemLoc::Host && target == MemLoc::Host) {
+      std::memcpy(c.outputs[0].data(), in.data(), n);
+    } else {
+      const hipMemcpyKind kind = src == MemLoc::Host ? hipMemcpyHostToDevice
+                                 : target == MemLoc::Host ? hipMemcpyDeviceToHost
+                                                          : hipMemcpyDeviceToDevice;
+      HIP_CHECK(hipMemcpyAsync(c.outputs[0].data(), in.data(), n, kind, st));
+    }
+    return OpResult::Ok;
+  }
+};
+
+MemLoc memcpy_target(const Json &j) {
+  const std::string &t = j.at("target").str();
+  if (t == "Host") return MemLoc::Host;
+  if (t == "Device") return MemLoc::Device;
+  throw std::invalid_argument("Invalid Target string: " + t);
+}
+
+class MemcpyFactory : public ISyncTaskFactory {
+ public:
+  InferResult infer(std::span<const TDesc> in, const Json &j) const override {
+    const MemLoc target = memcpy_target(j);
+    check(in.size() == 1, "MemcpyFactory", "Memcpy task must have exactly one input");
+    check(in[0].is_contiguous(), "MemcpyFactory", "Memcpy task input must be contiguous");
+    TDesc out = in[0];
+    out.mem_loc = target;
+    return sync_result({in[0]}, {out});
+  }
+  std::unique_ptr<ISyncTask> create(std::span<const TDesc> in, const Json &j, const SyncCreateCtx &ctx) const override {
+    (void)infer(in, j);
+    auto t = std::make_unique<MemcpyTask>();
+    t->st = ctx.stream;
+    t->target = memcpy_target(j);
+    return t;
+  }
+  std::unique_ptr<ISyncTask> update(std::unique_ptr<ISyncTask> old, std::span<const TDesc> in, const Json &j,
+                                    const SyncCreateCtx &ctx) const override {
+    (void)infer(in, j);
+    if (auto *t = dynamic_cast<MemcpyTask *>(old.get()); t && t->target == memcpy_target(j)) {
+      t->st = ctx.stream;
+      return old;
+    }
+    return create(in, j, ctx);
+  }
+};
+
 void register_hip_nodes(Registry &r) {
   r.register_async("BatchQueue", std::make_unique<BatchQueueFactory>());
   r.register_async("SlidingAverage", std::make_unique<SlidingAverageFactory>());
@@ -1826,6 +1886,7 @@ void register_hip_nodes(Registry &r) {
   r.register_sync("FFTShiftNp", std::make_unique<FFTShiftFactory>());
   r.register_sync("DopplerMoments", std::make_unique<DopplerMomentsFactory>());
   r.register_sync("Holofile", std::make_unique<HolofileFactory>());
+  r.register_sync("Memcpy", std::make_unique<MemcpyFactory>());
 }
 
 }  // namespace holoflow::core

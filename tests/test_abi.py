@@ -10,7 +10,7 @@ from holoflow_amd import _lib
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-HEADERS = ("hfx.h", "hfx_nodes.h", "hfx_serve.h", "hfx_holofile.h")
+HEADERS = ("hfx.h", "hfx_nodes.h", "hfx_serve.h", "hfx_holofile.h", "hfx_graph.h")
 
 
 def _declared():

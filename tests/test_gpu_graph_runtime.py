@@ -1,0 +1,111 @@
+"""The holoflow runtime on the GPU: the reference GraphBuilder's node chain compiled from
+graph JSON and driven section by section, the fused graph, threaded runs with metrics, and
+recompilation with reuse; every moment image against the FP64 oracle (rel-L2 <= 1e-5)."""
+import time
+
+import numpy as np
+import pytest
+
+from holoflow_amd.graph import Graph, doppler_graph, fused_doppler_graph
+from oracle import holofile as OH
+from oracle import oracle as O
+from oracle.synth import SynthParams, synth_frames
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+T, H, W = 32, 64, 64
+KB, KE = O.band_bins(6e3, 15e3, 37e3, T)
+F32 = lambda x: float(np.float32(x))  # noqa: E731  (the nodes hold lambda, dx, z as FP32)
+
+
+def _record(tmp_path, n_frames, seed=3, repeat=1):
+    frames, _ = synth_frames(SynthParams(H, W, 20.0, seed=seed, stream_id=4), 0, n_frames)
+    frames = np.concatenate([frames] * repeat)
+    p = tmp_path / "rec.holo"
+    p.write_bytes(OH.pack(frames, {"version": 1}))
+    return str(p), frames
+
+
+def _params(prop, both=False, z=0.3, **kw):
+    return O.PathParams(H, W, T, T, KB, KE, prop, lambda_m=F32(852e-9), dx_m=F32(12e-6), dy_m=F32(12e-6),
+                        z_m=F32(z), fftshift=True, both_sidebands=both, **kw)
+
+
+def _node_named(spec, kind, **params):
+    for name, n in sorted(spec["nodes"].items()):
+        if n["type"] == kind and all(n["params"].get(k) == v for k, v in params.items()):
+            return name
+    raise KeyError(kind)
+
+
+@pytest.mark.parametrize("moment,prop,tm,extra", [
+    ("M0", "fresnel", "RFFT", {}),
+    ("M1", "fresnel", "RFFT", {}),
+    ("M2", "asm", "FFT", {}),
+    ("M0", "fresnel", "RFFT", {"filter2d": {"r_inner": 2, "r_outer": 20, "s_inner": 1, "s_outer": 4},
+                               "flatfield_sigma": (2.0, 2.0)}),
+])
+def test_reference_graph_json_end_to_end(gpu, tmp_path, moment, prop, tm, extra):
+    path, frames = _record(tmp_path, 8 * T)
+    spec = doppler_graph(path, H, W, T, KB, KE, 8 * T, prop=prop, z_m=0.3, time_method=tm, moment=moment,
+                         fftshift=True, **extra)
+    g = Graph(spec)
+    desc = g.describe()
+    assert len(desc["sections"]) == 5          # split at the four asynchronous queues
+    # the image entering the batch mean (whose own output lands in the sliding average's
+    # ring slot and is handed over at the push)
+    final = next(e["from"] for e in spec["edges"] if e["to"] == _node_named(spec, "Mean", axis=[0]))
+    filt = tuple(extra["filter2d"].values()) if "filter2d" in extra else None
+    p = _params(prop, both=tm == "FFT", filt=filt, flatfield=extra.get("flatfield_sigma"))
+    mi = {"M0": 0, "M1": 1, "M2": 2}[moment]
+    outs = []
+    for step in range(15):                      # 8 windows fill the pre-batch queue at step 7
+        g.step()
+        if step >= 7:
+            outs.append(g.output(final).cpu().numpy())
+    for k, out in enumerate(outs):
+        ref = O.doppler_moments(frames[k * T:(k + 1) * T], p, "f64")[mi]
+        assert out.shape == (1, 1, H, W)
+        assert rel_l2(out[0, 0], ref) < 1e-5, f"window {k}"
+    u8 = g.output(_node_named(spec, "Conversion", target="U8"))
+    assert u8 is not None and u8.shape == (1, H, W)
+    g.close()
+
+
+def test_fused_graph_and_recompile(gpu, tmp_path):
+    path, frames = _record(tmp_path, 4 * T)
+    st = {"lambda": 852e-9, "dx": 12e-6, "z": 0.3, "prop": "fresnel", "fftshift": True}
+    spec = fused_doppler_graph(path, T, KB, KE, 4 * T, **st)
+    g = Graph(spec)
+    for k in range(4):
+        assert g.step() == 2
+        out = g.output("doppler_moments_0").cpu().numpy()
+        assert rel_l2(out, O.doppler_moments(frames[k * T:(k + 1) * T], _params("fresnel"), "f64")) < 1e-5
+    before = g.describe()
+    spec2 = fused_doppler_graph(path, T, KB, KE, 4 * T, **dict(st, z=0.35))
+    g2 = g.recompile(spec2)                     # same buffers, z-only task update
+    after = g2.describe()
+    assert after["device_bytes"] == before["device_bytes"] and after["host_bytes"] == before["host_bytes"]
+    g2.step()
+    out = g2.output("doppler_moments_0").cpu().numpy()
+    assert rel_l2(out, O.doppler_moments(frames[:T], _params("fresnel", z=0.35), "f64")) < 1e-5
+    g2.close()
+
+
+@pytest.mark.parametrize("load_kind", ["CPUCached", "GPUCached"])
+def test_threaded_run_metrics(gpu, tmp_path, load_kind):
+    path, frames = _record(tmp_path, T, repeat=4)   # every window holds the same frames
+    st = {"lambda": 852e-9, "dx": 12e-6, "z": 0.3, "prop": "fresnel", "fftshift": True}
+    spec = fused_doppler_graph(path, T, KB, KE, 4 * T, load_kind=load_kind, **st)
+    g = Graph(spec)
+    g.start()
+    time.sleep(1.0)
+    g.stop()
+    m = g.metrics()
+    assert set(m) == set(spec["nodes"])
+    assert all(v["sample_count"] > 0 for v in m.values()), m
+    assert m["doppler_moments_0"]["runs_per_second"] > 10
+    out = g.output("doppler_moments_0").cpu().numpy()
+    assert rel_l2(out, O.doppler_moments(frames[:T], _params("fresnel"), "f64")) < 1e-5
+    g.close()
