@@ -203,6 +203,52 @@ def stages_leg(args, np, torch, ring, n_windows, cfg, value):
     return res
 
 
+def graph_leg(args, np, torch, ring, value):
+    """The runtime on real graph JSON, section threads, device-cached recording of 4 windows
+    (looped): the fused graph (Holofile -> DopplerMoments) and the reference's node chain for
+    M0 (Conversion -> Reshape -> BatchQueue{8} -> RFFT -> Slice -> Copy -> DualReaderBatchQueue
+    -> FresnelDiffraction -> Mean -> FFTShiftNp -> Mean -> SlidingAverage -> Conversion U8).
+    Windows are T frames apart (the reference's BatchQueue{T, T}); a window's cost does not
+    depend on the stride, so windows/s compares with the fused plan's moment frames/s."""
+    import os
+    import tempfile
+
+    from holoflow_amd import graph as G
+    from holoflow_amd import holofile
+
+    n = 4 * T
+    host = ring[:n].cpu().numpy().view(np.uint16)
+    tmp = tempfile.mkdtemp()
+    path = os.path.join(tmp, "bench.holo")
+    holofile.write(path, host)
+    res = {}
+    st = {"lambda": 852e-9, "dx": 12e-6, "z": 0.45, "prop": "fresnel", "fftshift": True}
+    g = G.Graph(G.fused_doppler_graph(path, T, 11, 26, n, load_kind="GPUCached", **st))
+    g.start()
+    time.sleep(args.graph_seconds)
+    g.stop()
+    m = g.metrics()["doppler_moments_0"]
+    g.close()
+    res["fused_graph"] = {"windows_per_s": round(m["runs_per_second"], 1),
+                          "node_ms": round(m["average_duration_ms"], 4),
+                          "fraction_of_plan": round(m["runs_per_second"] / value, 4)}
+    spec = G.doppler_graph(path, H, W, T, 11, 26, n, z_m=0.45, fftshift=True, load_kind="GPUCached", moment="M0")
+    g = G.Graph(spec)
+    g.start()
+    time.sleep(args.graph_seconds)
+    g.stop()
+    mm = g.metrics()
+    g.close()
+    fres = mm["fresnel_diffraction_0"]
+    res["reference_chain_m0"] = {"windows_per_s": round(fres["runs_per_second"], 1),
+                                 "fresnel_node_ms": round(fres["average_duration_ms"], 4),
+                                 "rfft_node_ms_per_8_windows": round(mm["rfft_0"]["average_duration_ms"], 4),
+                                 "fraction_of_plan": round(fres["runs_per_second"] / value, 4)}
+    os.remove(path)
+    os.rmdir(tmp)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +261,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-cfg4", action="store_true")
     ap.add_argument("--no-stages", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--graph-seconds", type=float, default=3.0)
     ap.add_argument("--cfg4-frames", type=int, default=8192)
     ap.add_argument("--serving-streams", type=int, default=24)
     ap.add_argument("--serving-windows", type=int, default=128)
@@ -346,6 +394,11 @@ def main():
     if not args.no_stages:
         stages = stages_leg(args, np, torch, ring, n_windows, cfg, value)
         log(f"stages: {stages}")
+    # ---- the runtime on graph JSON (section threads)
+    graph_res = None
+    if not args.no_graph:
+        graph_res = graph_leg(args, np, torch, ring, value)
+        log(f"graph: {graph_res}")
     # ---- large single stream (2048^2, T=128, band split)
     big = None
     if not args.no_cfg4:
@@ -410,6 +463,7 @@ def main():
                        "window_period": round(window_period_ms, 4)},
         "mixed": mixed,
         "stages": stages,
+        "graph": graph_res,
         "cfg4": big,
         "serving": serve,
         "cpu_baseline": cpu,

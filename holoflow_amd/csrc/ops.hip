@@ -448,12 +448,95 @@ __global__ void k_dft_axis(DftAxisArgs a) {
   a.out[t] = make_float2(sr * a.scale, si * a.scale);
 }
 
+// Band DFT along one axis, tiled: a block owns 256 consecutive inner indices of one outer
+// index, so every load of x[o, j, i0 .. i0+255] is one coalesced row; each thread keeps
+// DFT_KC bins' accumulators and walks j once per chunk of bins.  The twiddles
+// w[m] = exp(-2 pi i m / n) are built in LDS (FP64 sincospi, rounded once); the phase index
+// (k j) mod n advances by k per sample (bins past the band advance by 0 and are not stored).
+// DT: 1 U16, 2 F32, 3 CF32 (compile-time, so real inputs carry no imaginary terms).
+constexpr int DFT_KC = 16, DFT_TILE = 256, DFT_MAXN = 4096;
+
+template <int DT>
+__global__ __launch_bounds__(DFT_TILE) void k_dft_axis_tiled(DftAxisArgs a, long long tiles) {
+  extern __shared__ float2 tw[];
+  for (int m = threadIdx.x; m < a.n; m += DFT_TILE) {
+    double s, c;
+    sincospi(2.0 * (double)m / (double)a.n, &s, &c);
+    tw[m] = make_float2((float)c, (float)(-s));
+  }
+  __syncthreads();
+  const long long o = blockIdx.x / tiles;
+  const long long i = (blockIdx.x % tiles) * DFT_TILE + threadIdx.x;
+  if (i >= a.inner) return;
+  const long long nk = a.ke - a.kb;
+  const long long base = o * (long long)a.n * a.inner + i;
+  for (int k0 = a.kb; k0 < a.ke; k0 += DFT_KC) {
+    float2 acc[DFT_KC];
+    int m[DFT_KC], step[DFT_KC];
+#pragma unroll
+    for (int c = 0; c < DFT_KC; ++c) {
+      acc[c] = make_float2(0.f, 0.f);
+      m[c] = 0;
+      step[c] = k0 + c < a.ke ? k0 + c : 0;
+    }
+#pragma unroll 2
+    for (int j = 0; j < a.n; ++j) {
+      const long long src = base + (long long)j * a.inner;
+      float xr, xi = 0.f;
+      if constexpr (DT == 1) {
+        xr = (float)reinterpret_cast<const uint16_t *>(a.in)[src];
+      } else if constexpr (DT == 2) {
+        xr = reinterpret_cast<const float *>(a.in)[src];
+      } else {
+        const float2 v = reinterpret_cast<const float2 *>(a.in)[src];
+        xr = v.x;
+        xi = v.y;
+      }
+      float2 w[DFT_KC];
+#pragma unroll
+      for (int c = 0; c < DFT_KC; ++c) w[c] = tw[m[c]];
+#pragma unroll
+      for (int c = 0; c < DFT_KC; ++c) {
+        if constexpr (DT == 3) {
+          acc[c].x += xr * w[c].x - xi * w[c].y;
+          acc[c].y += xr * w[c].y + xi * w[c].x;
+        } else {
+          acc[c].x += xr * w[c].x;
+          acc[c].y += xr * w[c].y;
+        }
+        m[c] += step[c];
+        m[c] = m[c] >= a.n ? m[c] - a.n : m[c];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < DFT_KC; ++c) {
+      if (k0 + c < a.ke) {
+        const long long k = k0 + c - a.kb;
+        a.out[(o * nk + k) * a.inner + i] = make_float2(acc[c].x * a.scale, acc[c].y * a.scale);
+      }
+    }
+  }
+}
+
 int launch_dft_axis(const void *in, int dtype, long long outer, int n, long long inner, int kb,
                     int ke, float scale, float2 *out, hipStream_t st) {
   DftAxisArgs a{in, dtype, outer, inner, n, kb, ke, scale, out};
   long long total = outer * (long long)(ke - kb) * inner;
   if (total == 0) return 0;
-  hipLaunchKernelGGL(k_dft_axis, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, a);
+  const long long tiles = (inner + DFT_TILE - 1) / DFT_TILE;
+  if (n <= DFT_MAXN && outer * tiles < (1LL << 31) && kb >= 0 && ke <= n) {
+    // the phase counters reach at most (k0 + c) + n - 1 < 2 n before their wrap
+    const dim3 grid((unsigned)(outer * tiles)), block(DFT_TILE);
+    const size_t lds = (size_t)n * sizeof(float2);
+    if (dtype == 1)
+      hipLaunchKernelGGL(k_dft_axis_tiled<1>, grid, block, lds, st, a, tiles);
+    else if (dtype == 2)
+      hipLaunchKernelGGL(k_dft_axis_tiled<2>, grid, block, lds, st, a, tiles);
+    else
+      hipLaunchKernelGGL(k_dft_axis_tiled<3>, grid, block, lds, st, a, tiles);
+  } else {
+    hipLaunchKernelGGL(k_dft_axis, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, a);
+  }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
