@@ -71,6 +71,7 @@ class PlanDesc(C.Structure):
         ("weight_fs_hz", C.c_double),
         ("flatfield_enabled", C.c_int32),
         ("flatfield_sigma_y", C.c_float), ("flatfield_sigma_x", C.c_float),
+        ("batch_windows", C.c_int32),
     ]
 
 

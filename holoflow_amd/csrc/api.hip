@@ -112,6 +112,7 @@ struct hfx_plan {
   float2 *lx = nullptr, *ly = nullptr;
   float2 *tf = nullptr;
   float *wf = nullptr, *wf2 = nullptr;
+  int nwin = 1;              // windows per front launch in hfx_moments_batch
   bool filter2d = false;     // Fresnel: Filter2D on the band planes before the lens
   SpectralFilter f2d;        // its tables; workspace = planes_blk
   hfx_flatfield *ff = nullptr;  // Flatfield on the moment images
@@ -158,7 +159,7 @@ static hfx_status spectral_setup(SpectralFilter &f, int64_t batch, int H, int W,
                                  float2 *borrowed_ws = nullptr);
 static hfx_status spectral_run(const SpectralFilter &f, const void *in, bool in_real, void *out, bool out_real,
                                hipStream_t st);
-static hfx_status window_back_moments(hfx_plan *p, float *out, hipStream_t st);
+static hfx_status window_back_moments(hfx_plan *p, float2 *planes, float *out, hipStream_t st);
 
 extern "C" {
 
@@ -265,7 +266,11 @@ hfx_status hfx_plan_create(const hfx_plan_desc *dp, hfx_stream_t stream, hfx_pla
   p->fused_row = !p->filter2d && p->OH == p->H && p->OW == p->W && hfx::dft_row_supported(p->W, p->nbc, d.window);
   // natural-layout planes: only the stand-alone time DFT needs them
   HFX_TRY(A.alloc(&p->planes_nat, p->fused_row ? 1 : (size_t)p->n_planes * p->npix));
-  HFX_TRY(A.alloc(&p->planes_blk, (size_t)p->n_planes * p->onpix));
+  if (d.batch_windows > 1 && p->fused_row &&
+      hfx::dft_front_pipelined(p->W, p->nbc, d.window, p->n_planes, d.precision))
+    p->nwin = std::min<int>(d.batch_windows, hfx::HFX_MAX_WIN);
+  // one blocked plane workspace per window of a front launch
+  HFX_TRY(A.alloc(&p->planes_blk, (size_t)p->nwin * p->n_planes * p->onpix));
   HFX_TRY(A.alloc(&p->tw_w, p->OW));
   HFX_TRY(A.alloc(&p->tw_h, p->OH));
   HFX_TRY(A.alloc(&p->wm32, (size_t)p->wm_rows * d.window));
@@ -426,13 +431,13 @@ static hfx_status window_front(hfx_plan *p, const uint16_t *ring, int64_t ring_f
 
 // Back of the schedule: column pass (Fresnel: with |.| and the moment reduction; ASM: with
 // the transfer function, then the inverse row pass with the reduction) into `out`.
-static hfx_status window_back(hfx_plan *p, float *out, hipStream_t st) {
-  HFX_TRY(window_back_moments(p, p->ff ? p->mom_tmp : out, st));
+static hfx_status window_back(hfx_plan *p, float *out, hipStream_t st, float2 *planes = nullptr) {
+  HFX_TRY(window_back_moments(p, planes ? planes : p->planes_blk, p->ff ? p->mom_tmp : out, st));
   if (p->ff) return hfx_flatfield_execute(p->ff, p->mom_tmp, out, reinterpret_cast<hfx_stream_t>(st));
   return HFX_OK;
 }
 
-static hfx_status window_back_moments(hfx_plan *p, float *out, hipStream_t st) {
+static hfx_status window_back_moments(hfx_plan *p, float2 *planes, float *out, hipStream_t st) {
   const long long on = p->onpix;
   hfx::MomentArgs mom{};
   for (int i = 0; i < p->n_out; ++i) mom.out[i] = out + i * on;
@@ -443,7 +448,7 @@ static hfx_status window_back_moments(hfx_plan *p, float *out, hipStream_t st) {
   mom.shift = (p->d.flags & HFX_FLAG_FFTSHIFT) ? 1 : 0;
 
   hfx::PassArgs c = base_pass(p->OH, p->OW, p->n_planes, p->blk_c);
-  c.in = p->planes_blk;
+  c.in = planes;
   c.in_mode = hfx::IN_BLK;
   c.tw = p->tw_h;
   if (p->d.prop == HFX_PROP_FRESNEL) {
@@ -452,11 +457,11 @@ static hfx_status window_back_moments(hfx_plan *p, float *out, hipStream_t st) {
     if (hfx::launch_col_pass(c, st) != 0) return fail(HFX_E_HIP, "column pass launch failed");
   } else {
     c.mid_tf = p->tf;
-    c.out = p->planes_blk;
+    c.out = planes;
     c.out_mode = hfx::OUT_BLK;
     if (hfx::launch_col_pass(c, st) != 0) return fail(HFX_E_HIP, "column pass launch failed");
     hfx::PassArgs r2 = base_pass(p->OH, p->OW, p->n_planes, p->blk_c);
-    r2.in = p->planes_blk;
+    r2.in = planes;
     r2.in_mode = hfx::IN_BLK;
     r2.inv = 1;
     r2.out_mode = hfx::OUT_MOMENTS;
@@ -500,10 +505,26 @@ hfx_status hfx_moments_batch(hfx_plan *p, const uint16_t *ring, int64_t ring_fra
   HFX_TRY(check_ring(p, ring, ring_frames, first_frame));
   if (!out || n_windows < 0) return fail(HFX_E_INVALID, "invalid output");
   if (out_stride < (int64_t)p->n_out * p->onpix) return fail(HFX_E_INVALID, "out_stride too small");
-  for (int w = 0; w < n_windows; ++w) {
-    int64_t f = (first_frame + (int64_t)w * p->d.stride) % ring_frames;
-    HFX_TRY(window_moments(p, ring, ring_frames, f, out + w * out_stride, S(stream)));
+  hipStream_t st = S(stream);
+  auto frame_of = [&](int w) { return (first_frame + (int64_t)w * p->d.stride) % ring_frames; };
+  if (p->nwin > 1) {
+    // nwin windows per front launch (the pipelined kernel runs on across their rows), then
+    // the back of each window from its own plane workspace
+    const long long pstride = (long long)p->n_planes * p->onpix;
+    for (int w = 0; w < n_windows; w += p->nwin) {
+      const int k = std::min(p->nwin, n_windows - w);
+      hfx::DftLaunch L = dft_launch(p, ring, ring_frames, frame_of(w), p->planes_blk);
+      L.nwin = k;
+      for (int j = 0; j < k; ++j) L.firsts[j] = frame_of(w + j);
+      L.planes_stride = pstride;
+      if (hfx::launch_time_dft_row(L, p->OH, p->blk_c, p->n_planes, p->tw_w, st) != 0)
+        return fail(HFX_E_HIP, "multi-window front launch failed");
+      for (int j = 0; j < k; ++j) HFX_TRY(window_back(p, out + (w + j) * out_stride, st, p->planes_blk + j * pstride));
+    }
+    return HFX_OK;
   }
+  for (int w = 0; w < n_windows; ++w)
+    HFX_TRY(window_moments(p, ring, ring_frames, frame_of(w), out + w * out_stride, st));
   return HFX_OK;
 }
 

@@ -31,6 +31,10 @@ struct DftArgs {
   // k_dft_row only
   int H, blk_c, n_planes;
   const float2 *tw;        // W-point FFT twiddles
+  // k_front only: windows of one launch (first frames, planes at planes + w * planes_stride)
+  int nwin;
+  long long firsts[HFX_MAX_WIN];
+  long long planes_stride;
 };
 
 // value of computed bin `bin` for 4 pixels, with the centring offset restored on bin 0
@@ -238,7 +242,10 @@ __global__ __launch_bounds__(row_threads<W>()) void k_dft_row(DftArgs a) {
 // through a K-step register ring that runs on across rows (DftStream); four consumer waves
 // share the planes.  FP32 only, for T in {32, 64, 128}.
 template <int W> constexpr int front_pw() { return W / 128; }
-template <int W> constexpr int front_cw() { return 4; }
+#ifndef HFX_FRONT_CW
+#define HFX_FRONT_CW 4
+#endif
+template <int W> constexpr int front_cw() { return HFX_FRONT_CW; }
 template <int W> constexpr int front_threads() { return (front_pw<W>() + front_cw<W>()) * 64; }
 #ifndef HFX_FRONT_RING
 #define HFX_FRONT_RING 8
@@ -270,16 +277,25 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
   const bool lens = a.lx != nullptr;
   const int G = gridDim.x;
   const int y0 = xcd_tile(blockIdx.x, G);
-  const int nr = y0 < a.H ? (a.H - 1 - y0) / G + 1 : 0;
+  const int nrw = y0 < a.H ? (a.H - 1 - y0) / G + 1 : 0;  // rows of one window
+  const int nr = nrw * a.nwin;                             // rows of all windows, window-major
 
   f4 acc[TPW][NBT][4];
   DftStream<NBT, TPW, T, front_ring(T)> stream;
   auto px0 = [&](int y) { return (long long)y * W + rw * 64 * TPW; };
+  auto row_y = [&](int i) { return y0 + (nrw ? i % nrw : 0) * G; };
+  auto ring_of = [&](int i) {
+    DftRing r = a.r;
+    r.first = a.firsts[nrw ? i / nrw : 0];
+    return r;
+  };
 
-  // producers: the row's band spectra for their 128 pixels
-  auto dft_row = [&](int y, int lane) {
-    const bool has_next = y + G < a.H;
-    stream.row(a.r, reinterpret_cast<const float *>(wmat), px0(y), px0(has_next ? y + G : y), has_next, lane, acc);
+  // producers: the band spectra of row i for their 128 pixels
+  auto dft_row = [&](int i, int lane) {
+    const bool has_next = i + 1 < nr;
+    const int j = has_next ? i + 1 : i;
+    stream.row(ring_of(i), ring_of(j), reinterpret_cast<const float *>(wmat), px0(row_y(i)), px0(row_y(j)), has_next,
+               lane, acc);
   };
   // producers: MFMA registers -> padded LDS rows (lens applied for Fresnel)
   auto scatter = [&](int y, int lane) {
@@ -324,7 +340,9 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
     }
   };
   // consumers: one wave per plane row: forward row FFT, then the column-blocked store
-  auto transform = [&](int y, int lane) {
+  auto transform = [&](int i, int lane) {
+    const int y = row_y(i);
+    float2 *planes = a.planes + (nrw ? i / nrw : 0) * a.planes_stride;
     for (int q = rw; q < a.n_planes; q += front_cw<W>()) {
       float2 *ws = scr + q * F::SCR;
       float2 v[E];
@@ -334,8 +352,8 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int x = lane + 64 * m;
-        a.planes[(long long)q * plane_elems + (long long)(x / a.blk_c) * a.H * a.blk_c +
-                 (long long)y * a.blk_c + (x % a.blk_c)] = v[m];
+        planes[(long long)q * plane_elems + (long long)(x / a.blk_c) * a.H * a.blk_c + (long long)y * a.blk_c +
+               (x % a.blk_c)] = v[m];
       }
     }
   };
@@ -346,19 +364,18 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
   if (producer) {
     const int lane0 = tid & 63;
     if (nr > 0) {
-      stream.start(a.r, px0(y0), lane0);
-      dft_row(y0, lane0);
+      stream.start(ring_of(0), px0(y0), lane0);
+      dft_row(0, lane0);
       scatter(y0, lane0);
     }
     __syncthreads();
     for (int i = 0; i < nr; ++i) {
-      const int y = y0 + i * G;
       int lane;
       asm volatile("v_mov_b32 %0, %1" : "=v"(lane) : "v"(tid & 63));
-      if (i + 1 < nr) dft_row(y + G, lane);
-      __syncthreads();  // consumers are done with row y's planes
-      if (i + 1 < nr) scatter(y + G, lane);
-      __syncthreads();  // row y + G's planes are in LDS
+      if (i + 1 < nr) dft_row(i + 1, lane);
+      __syncthreads();  // consumers are done with row i's planes
+      if (i + 1 < nr) scatter(row_y(i + 1), lane);
+      __syncthreads();  // row i + 1's planes are in LDS
     }
   } else {
     __syncthreads();
@@ -366,7 +383,7 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
       // opaque per-row lane index: keeps loop-invariant LDS addresses out of registers
       int lane;
       asm volatile("v_mov_b32 %0, %1" : "=v"(lane) : "v"(tid & 63));
-      transform(y0 + i * G, lane);
+      transform(i, lane);
       __syncthreads();
       __syncthreads();
     }
@@ -395,6 +412,10 @@ static DftArgs make_args(const DftLaunch &L) {
     default: a.wm_hi = L.wm_f16hi; break;
   }
   a.wm_lo = L.wm_f16lo;
+  a.nwin = L.nwin > 0 ? L.nwin : 1;
+  for (int w = 0; w < HFX_MAX_WIN; ++w) a.firsts[w] = w < L.nwin ? L.firsts[w] : L.first;
+  a.firsts[0] = L.first;
+  a.planes_stride = L.planes_stride;
   return a;
 }
 
@@ -432,6 +453,12 @@ int launch_time_dft(const DftLaunch &L, hipStream_t st) {
 
 bool dft_row_supported(int W, int nbc, int T) {
   return (W == 512 || W == 768 || W == 1024) && (2 * nbc + 15) / 16 <= 2 && T <= 128;
+}
+
+bool dft_front_pipelined(int W, int nbc, int T, int n_planes, int precision) {
+  if (!dft_row_supported(W, nbc, T) || precision != 0 || !(T == 32 || T == 64 || T == 128)) return false;
+  const int group = W == 512 ? row_group<512>() : W == 768 ? row_group<768>() : row_group<1024>();
+  return n_planes <= group;
 }
 
 template <int W, int NBT, int MODE> static int launch_row(const DftArgs &a, hipStream_t st) {
@@ -476,6 +503,8 @@ template <int W> static int launch_row_w(const DftArgs &a, int nbt, int mode, hi
 int launch_time_dft_row(const DftLaunch &L, int H, int blk_c, int n_planes, const float2 *tw,
                         hipStream_t st) {
   if (!dft_row_supported(L.W, L.nbc, L.T)) return -2;
+  if (L.nwin > 1 && (L.nwin > HFX_MAX_WIN || !dft_front_pipelined(L.W, L.nbc, L.T, n_planes, L.precision)))
+    return -6;
   DftArgs a = make_args(L);
   a.H = H;
   a.blk_c = blk_c;

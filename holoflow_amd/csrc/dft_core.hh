@@ -276,10 +276,11 @@ template <int NBT, int TPW, int T, int K> struct DftStream {
 #pragma unroll
     for (int k = 0; k < K; ++k) load(r, p0w, lane, k, k);
   }
-  // D = Wm * X for the row at p0w (operand matrix lw with pitch T + 4 floats); the ring
-  // refills from this row, then from the row at p0w_next when has_next.
-  __device__ __forceinline__ void row(const DftRing &r, const float *lw, long long p0w, long long p0w_next,
-                                      bool has_next, int lane, f4 (&acc)[TPW][NBT][4]) {
+  // D = Wm * X for the row at p0w of window r (operand matrix lw with pitch T + 4 floats);
+  // the ring refills from this row, then from the row at p0w_next of window r_next when
+  // has_next.
+  __device__ __forceinline__ void row(const DftRing &r, const DftRing &r_next, const float *lw, long long p0w,
+                                      long long p0w_next, bool has_next, int lane, f4 (&acc)[TPW][NBT][4]) {
 #pragma unroll
     for (int i = 0; i < TPW; ++i)
 #pragma unroll
@@ -296,7 +297,7 @@ template <int NBT, int TPW, int T, int K> struct DftStream {
       if (s + K < STEPS)
         load(r, p0w, lane, s + K, slot);
       else if (has_next)
-        load(r, p0w_next, lane, s + K - STEPS, slot);
+        load(r_next, p0w_next, lane, s + K - STEPS, slot);
       const int n = 4 * s + (lane >> 4);
 #pragma unroll
       for (int bt = 0; bt < NBT; ++bt) {

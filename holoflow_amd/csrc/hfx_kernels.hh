@@ -32,9 +32,16 @@ int launch_flatfield_spatial(const float *in, float *out, float *tmp, long long 
                              const float *kx, int rx, const float *ky, int ry, hipStream_t st);
 
 // ---- time DFT (dft.hip)
+constexpr int HFX_MAX_WIN = 4;  // windows per pipelined front launch
+
 struct DftLaunch {
   const uint16_t *ring;
   long long ring_frames, first;
+  // pipelined front only: windows 1 .. nwin-1 start at firsts[w] and write their planes at
+  // planes + w * planes_stride (nwin = 1: just `first` and `planes`)
+  int nwin;
+  long long firsts[HFX_MAX_WIN];
+  long long planes_stride;
   int T;
   long long npix;
   int W;
@@ -48,6 +55,8 @@ struct DftLaunch {
 int launch_time_dft(const DftLaunch &L, hipStream_t st);
 // Fused time DFT + forward row FFT into the column-blocked layout (L.planes is blocked).
 bool dft_row_supported(int W, int nbc, int T);
+// true when launch_time_dft_row runs the pipelined front kernel (and so accepts nwin > 1)
+bool dft_front_pipelined(int W, int nbc, int T, int n_planes, int precision);
 int launch_time_dft_row(const DftLaunch &L, int H, int blk_c, int n_planes, const float2 *tw,
                         hipStream_t st);
 

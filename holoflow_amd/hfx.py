@@ -53,6 +53,7 @@ class MomentConfig:
     filt: Optional[tuple] = None           # (r_inner, r_outer, s_inner, s_outer)
     split_bin: int = 0
     flatfield: Optional[tuple] = None      # (sigma_y, sigma_x)
+    batch_windows: int = 1                 # windows per front launch in moments_batch
 
     def to_desc(self) -> _lib.PlanDesc:
         d = _lib.PlanDesc()
@@ -73,6 +74,7 @@ class MomentConfig:
             d.r_inner, d.r_outer, d.s_inner, d.s_outer = (float(v) for v in self.filt)
         d.split_bin = self.split_bin
         d.weight_fs_hz = self.weight_fs_hz
+        d.batch_windows = self.batch_windows
         if self.flatfield is not None:
             d.flatfield_enabled = 1
             d.flatfield_sigma_y, d.flatfield_sigma_x = (float(v) for v in self.flatfield)

@@ -79,6 +79,10 @@ typedef struct hfx_plan_desc {
   int32_t flatfield_enabled; /* Flatfield on every moment image after the fftshift
                                 (graph_builder.cc:637-641) */
   float flatfield_sigma_y, flatfield_sigma_x;
+  int32_t batch_windows;     /* hfx_moments_batch: windows per front launch (1..4; more keep
+                                the pipelined front kernel busy across window boundaries at
+                                the cost of one plane workspace each; FP32 Fresnel/ASM
+                                without padding or Filter2D, else 1) */
 } hfx_plan_desc;
 
 typedef struct hfx_plan_info {

@@ -128,6 +128,27 @@ def test_batch_matches_single_windows(gpu):
         assert torch.equal(batch[w], single)
 
 
+@pytest.mark.parametrize("prop,W,nwin", [("fresnel", 1024, 4), ("fresnel", 512, 3), ("asm", 1024, 2)])
+def test_multi_window_front_matches_single_windows(gpu, prop, W, nwin):
+    """batch_windows > 1: one pipelined front launch runs across several windows' rows; every
+    window must equal its own single-window result bit for bit (7 windows: a partial last
+    group; first frame near the ring's end: windows wrap)."""
+    from holoflow_amd import hfx
+
+    T, S, F = 64, 16, 200
+    p = O.PathParams(64, W, T, S, 11, 26, prop, 852e-9, 12e-6, 12e-6, 0.45, fftshift=True)
+    ring = _frames(64, W, F)
+    cfg = _cfg(p)
+    cfg.batch_windows = nwin
+    multi = hfx.MomentPlan(cfg)
+    single = hfx.MomentPlan(_cfg(p))
+    first = F - 40
+    batch = multi.moments_batch(ring, first, 7)
+    for w in range(7):
+        ref = single.moments(ring, (first + S * w) % F)
+        assert torch.equal(batch[w], ref), f"window {w}"
+
+
 @pytest.mark.parametrize("prop,W", [("fresnel", 512), ("asm", 64)])
 def test_stages_compose_to_moments(gpu, prop, W):
     # stage 0 then stage 1 is hfx_moments (fused front kernel at 512, unfused at 64)
