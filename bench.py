@@ -115,7 +115,7 @@ def cfg4_leg(args, np, torch):
     return res
 
 
-def serving_leg(args, np, torch, load=0.7, slots=16):
+def serving_leg(args, np, torch, load=0.7, slots=16, policy="fifo"):
     """Mixed-stream load: sizes {512^2, 768^2, 1024^2, 2048x512}, T in {32, 64, 128}, S = T/4,
     z ~ U(0.2, 0.6) m, f_s ~ U(20, 75) kHz, ASM.  Each stream's windows arrive as a Poisson
     process whose rate is set so that `slots` concurrent streams offer `load` of the GPU
@@ -131,11 +131,12 @@ def serving_leg(args, np, torch, load=0.7, slots=16):
     gaps = rng.exponential(float(np.mean(life)) / slots, len(specs))
     arrivals = np.concatenate([[0.0], np.cumsum(gaps[1:])])
     res = serving.run_workload(specs, arrivals, rates, args.serving_windows,
-                               server=serving.server_desc(max_streams=slots, n_lanes=4), seed=11,
+                               server=serving.server_desc(max_streams=slots, n_lanes=4, policy=policy), seed=11,
                                time_limit_s=240.0)
     summary = serving.summarize(specs, res)
     summary.update({
         "offered_gpu_load_at_16_streams": load,
+        "policy": policy,
         "window_cost_us_alone_p50": round(float(np.median(costs)), 1),
         "note": "latency = ingest of the window's last frame -> moments complete (HIP events); "
                 "window period = stride / f_s of the stream",

@@ -168,7 +168,11 @@ SIGNATURES = {
 
 class ServerDesc(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_int64), ("hbm_reserve_bytes", C.c_int64),
-                ("max_streams", C.c_int32), ("n_lanes", C.c_int32), ("use_graphs", C.c_int32)]
+                ("max_streams", C.c_int32), ("n_lanes", C.c_int32), ("use_graphs", C.c_int32),
+                ("policy", C.c_int32), ("lane_depth", C.c_int32)]
+
+
+SCHED_FIFO, SCHED_EDF = 0, 1
 
 
 class StreamDesc(C.Structure):

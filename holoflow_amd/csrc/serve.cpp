@@ -19,6 +19,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <memory>
 #include <random>
 #include <string>
@@ -96,6 +97,7 @@ struct Stream {
   double *phase = nullptr;
   std::vector<hipGraphExec_t> graphs;
   std::vector<hipEvent_t> ev_ready, ev_launch, ev_done;
+  std::vector<double> ready_t;  // host time (s) at which the window of each position became ready
   int64_t frames_in = 0, launched = 0, retired = 0;
 
   int64_t ready_windows() const { return frames_in >= g.T ? (frames_in - g.T) / g.S + 1 : 0; }
@@ -129,7 +131,10 @@ struct hfx_server {
   std::vector<hipStream_t> lanes;
   std::vector<std::unique_ptr<Stream>> slots;
   std::vector<int> lane_load;
+  std::vector<std::deque<std::pair<int32_t, int64_t>>> lane_windows;  // EDF: (stream, window) per lane
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   int64_t bytes_held = 0, windows_launched = 0, windows_retired = 0;
+  double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 };
 
 namespace {
@@ -165,6 +170,7 @@ hfx_status build_stream(hfx_server *s, Stream &st) {
   SRV_HIP(hipMemsetAsync(st.phase, 0, (size_t)g.npix * 8, cs));
   SRV_TRY(hfx_synth_frames(&st.d.camera, st.pool, 0, g.P, st.phase, cs));
   SRV_HIP(hipStreamSynchronize(cs));
+  st.ready_t.assign(g.n_pos, 0.0);
   st.ev_ready.assign(g.n_pos, nullptr);
   st.ev_launch.assign(g.n_pos, nullptr);
   st.ev_done.assign(g.n_pos, nullptr);
@@ -193,10 +199,10 @@ hfx_status build_stream(hfx_server *s, Stream &st) {
   return HFX_OK;
 }
 
-hfx_status launch_window(hfx_server *s, Stream &st) {
+hfx_status launch_window(hfx_server *s, Stream &st, int lane_idx) {
   const int64_t w = st.launched;
   const int64_t p = w % st.g.n_pos;
-  hipStream_t lane = s->lanes[st.lane];
+  hipStream_t lane = s->lanes[lane_idx];
   SRV_HIP(hipStreamWaitEvent(lane, st.ev_ready[p], 0));
   SRV_HIP(hipEventRecord(st.ev_launch[p], lane));
   if (s->d.use_graphs) {
@@ -221,6 +227,8 @@ void hfx_server_desc_init(hfx_server_desc *d) {
   d->max_streams = 16;
   d->n_lanes = 4;
   d->use_graphs = 1;
+  d->policy = HFX_SCHED_FIFO;
+  d->lane_depth = 1;
 }
 
 void hfx_stream_desc_init(hfx_stream_desc *d, int32_t height, int32_t width) {
@@ -246,6 +254,7 @@ hfx_status hfx_server_create(const hfx_server_desc *d, hfx_server **out) {
   if (d->max_streams < 1 || d->max_streams > 1024) return fail(HFX_E_INVALID, "max_streams must be in [1, 1024]");
   if (d->n_lanes < 1 || d->n_lanes > 16) return fail(HFX_E_INVALID, "n_lanes must be in [1, 16]");
   if (d->hbm_budget_bytes < 0 || d->hbm_reserve_bytes < 0) return fail(HFX_E_INVALID, "negative memory size");
+  if (d->policy != HFX_SCHED_FIFO && d->policy != HFX_SCHED_EDF) return fail(HFX_E_INVALID, "unknown policy");
   auto s = std::make_unique<hfx_server>();
   s->d = *d;
   SRV_HIP(hipGetDevice(&s->device));
@@ -254,6 +263,7 @@ hfx_status hfx_server_create(const hfx_server_desc *d, hfx_server **out) {
   s->lanes.assign(d->n_lanes, nullptr);
   for (auto &l : s->lanes) SRV_HIP(hipStreamCreateWithFlags(&l, hipStreamNonBlocking));
   s->lane_load.assign(d->n_lanes, 0);
+  s->lane_windows.resize(d->n_lanes);
   s->slots.resize(d->max_streams);
   *out = s.release();
   return HFX_OK;
@@ -324,20 +334,70 @@ hfx_status hfx_server_ingest(hfx_server *s, int32_t stream_id, int64_t n_frames)
   const int64_t old_ready = st->ready_windows();
   SRV_TRY(copy_frames(s, *st, st->frames_in, n_frames));
   st->frames_in = F;
-  for (int64_t w = old_ready; w < st->ready_windows(); ++w)
+  const double t_ready = s->now();
+  for (int64_t w = old_ready; w < st->ready_windows(); ++w) {
     SRV_HIP(hipEventRecord(st->ev_ready[w % g.n_pos], s->ingest));
+    st->ready_t[w % g.n_pos] = t_ready;
+  }
+  return HFX_OK;
+}
+
+// true while the stream's last launched window has not completed on the GPU
+static bool window_on_gpu(const Stream &st, int64_t w) {
+  return hipEventQuery(st.ev_done[w % st.g.n_pos]) == hipErrorNotReady;
+}
+
+static hfx_status schedule_edf(hfx_server *s, int32_t &n) {
+  // drop finished windows from the lanes' queues
+  for (auto &q : s->lane_windows) {
+    while (!q.empty()) {
+      const Stream *st = stream_of(s, q.front().first);
+      if (st && q.front().second < st->launched && window_on_gpu(*st, q.front().second)) break;
+      q.pop_front();
+    }
+  }
+  const int depth = std::max(1, s->d.lane_depth);
+  for (;;) {
+    int lane = -1;
+    for (int l = 0; l < (int)s->lanes.size(); ++l)
+      if ((int)s->lane_windows[l].size() < depth && (lane < 0 || s->lane_windows[l].size() < s->lane_windows[lane].size()))
+        lane = l;
+    if (lane < 0) break;
+    Stream *best = nullptr;
+    double best_deadline = 0.0;
+    for (auto &up : s->slots) {
+      if (!up) continue;
+      Stream &st = *up;
+      if (st.launched >= st.ready_windows()) continue;
+      if (st.launched > 0 && window_on_gpu(st, st.launched - 1)) continue;  // one window in flight
+      const double period = (double)st.g.S / (st.d.plan.fs_hz > 0 ? st.d.plan.fs_hz : 1.0);
+      const double deadline = st.ready_t[st.launched % st.g.n_pos] + period;
+      if (!best || deadline < best_deadline) {
+        best = &st;
+        best_deadline = deadline;
+      }
+    }
+    if (!best) break;
+    s->lane_windows[lane].emplace_back(best->id, best->launched);
+    SRV_TRY(launch_window(s, *best, lane));
+    ++n;
+  }
   return HFX_OK;
 }
 
 hfx_status hfx_server_schedule(hfx_server *s, int32_t *launched) {
   if (!s) return fail(HFX_E_INVALID, "null server");
   int32_t n = 0;
-  for (auto &up : s->slots) {
-    if (!up) continue;
-    Stream &st = *up;
-    while (st.launched < st.ready_windows()) {
-      SRV_TRY(launch_window(s, st));
-      ++n;
+  if (s->d.policy == HFX_SCHED_EDF) {
+    SRV_TRY(schedule_edf(s, n));
+  } else {
+    for (auto &up : s->slots) {
+      if (!up) continue;
+      Stream &st = *up;
+      while (st.launched < st.ready_windows()) {
+        SRV_TRY(launch_window(s, st, st.lane));
+        ++n;
+      }
     }
   }
   if (launched) *launched = n;
@@ -385,7 +445,9 @@ hfx_status hfx_server_release(hfx_server *s, int32_t stream_id) {
   Stream *st = stream_of(s, stream_id);
   if (!st) return fail(HFX_E_INVALID, "unknown stream id");
   SRV_HIP(hipStreamSynchronize(s->ingest));
-  SRV_HIP(hipStreamSynchronize(s->lanes[st->lane]));
+  for (auto l : s->lanes) SRV_HIP(hipStreamSynchronize(l));  // EDF may have used any lane
+  for (auto &q : s->lane_windows)
+    q.erase(std::remove_if(q.begin(), q.end(), [&](const auto &e) { return e.first == stream_id; }), q.end());
   s->lane_load[st->lane]--;
   s->bytes_held -= st->g.total();
   st->free_all();

@@ -117,12 +117,15 @@ def window_cost_us(spec: StreamSpec, reps: int = 20) -> float:
     return cost
 
 
-def server_desc(max_streams=16, n_lanes=4, budget_bytes=0, reserve_bytes=8 << 30, use_graphs=True):
+def server_desc(max_streams=16, n_lanes=4, budget_bytes=0, reserve_bytes=8 << 30, use_graphs=True,
+                policy="fifo", lane_depth=1):
     d = _lib.ServerDesc()
     lib().hfx_server_desc_init(C.byref(d))
     d.max_streams, d.n_lanes = max_streams, n_lanes
     d.hbm_budget_bytes, d.hbm_reserve_bytes = budget_bytes, reserve_bytes
     d.use_graphs = int(use_graphs)
+    d.policy = {"fifo": _lib.SCHED_FIFO, "edf": _lib.SCHED_EDF}[policy]
+    d.lane_depth = lane_depth
     return d
 
 

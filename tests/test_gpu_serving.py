@@ -31,13 +31,15 @@ def _eager(spec: serving.StreamSpec, n_windows: int) -> np.ndarray:
     return np.stack(outs)
 
 
-def test_alone_and_in_mix_results_identical(gpu):
+@pytest.mark.parametrize("policy", ["fifo", "edf"])
+def test_alone_and_in_mix_results_identical(gpu, policy):
     specs = serving.mixed_streams(6, seed=11, **SMALL)
     k, wps = 2, 12
     arrivals = np.concatenate([[0.0], np.cumsum(np.random.default_rng(1).exponential(0.002, 5))])
     rates = [3000.0] * len(specs)
     mix = serving.run_workload(specs, arrivals, rates, wps, capture_stream=k,
-                               server=serving.server_desc(max_streams=4, n_lanes=3), time_limit_s=60)
+                               server=serving.server_desc(max_streams=4, n_lanes=3, policy=policy),
+                               time_limit_s=60)
     assert mix.windows_retired == len(specs) * wps
     assert mix.max_concurrent >= 2
     alone = serving.run_workload([specs[k]], [0.0], [rates[k]], wps, capture_stream=0,
