@@ -72,6 +72,7 @@ class PlanDesc(C.Structure):
         ("flatfield_enabled", C.c_int32),
         ("flatfield_sigma_y", C.c_float), ("flatfield_sigma_x", C.c_float),
         ("batch_windows", C.c_int32),
+        ("batch_lanes", C.c_int32),
     ]
 
 

@@ -200,18 +200,16 @@ __global__ __launch_bounds__(row_threads<W>()) void k_dft_row(DftArgs a) {
     // one wave per plane row of the group: forward row FFT, then the column-blocked store
     auto transform = [&](int q0) {
       const int q1 = min(q0 + GB, a.n_planes);
+      const long long row_off = blk_lane_offset(lane, a.H, a.blk_c) + (long long)y * a.blk_c;
       for (int q = q0 + wave; q < q1; q += NW) {
         float2 *ws = scr + (q - q0) * F::SCR;
         float2 v[E];
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = ws[F::pad(lane + 64 * m)];
         F::template run<false>(v, ws, twl, lane);
+        float2 *dst = a.planes + (long long)q * plane_elems + row_off;
 #pragma unroll
-        for (int m = 0; m < E; ++m) {
-          const int x = lane + 64 * m;
-          a.planes[(long long)q * plane_elems + (long long)(x / a.blk_c) * a.H * a.blk_c +
-                   (long long)y * a.blk_c + (x % a.blk_c)] = v[m];
-        }
+        for (int m = 0; m < E; ++m) dst[(long long)m * 64 * a.H] = v[m];
       }
     };
     if constexpr (!MULTI) {
@@ -343,18 +341,16 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
   auto transform = [&](int i, int lane) {
     const int y = row_y(i);
     float2 *planes = a.planes + (nrw ? i / nrw : 0) * a.planes_stride;
+    const long long row_off = blk_lane_offset(lane, a.H, a.blk_c) + (long long)y * a.blk_c;
     for (int q = rw; q < a.n_planes; q += front_cw<W>()) {
       float2 *ws = scr + q * F::SCR;
       float2 v[E];
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = ws[F::pad(lane + 64 * m)];
       F::template run<false>(v, ws, twl, lane);
+      float2 *dst = planes + (long long)q * plane_elems + row_off;
 #pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const int x = lane + 64 * m;
-        planes[(long long)q * plane_elems + (long long)(x / a.blk_c) * a.H * a.blk_c + (long long)y * a.blk_c +
-               (x % a.blk_c)] = v[m];
-      }
+      for (int m = 0; m < E; ++m) dst[(long long)m * 64 * a.H] = v[m];
     }
   };
 

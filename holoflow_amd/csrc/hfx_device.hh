@@ -153,4 +153,13 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
   return (b & 7) * (n8 >> 3) + (b >> 3);
 }
 
+// Column-blocked layout of an H x W plane with column blocks of C in {1, 2, 4}: element
+// (y, x) sits at (x / C) * H * C + y * C + x % C.  For the wave-owned row x = lane + 64 m
+// (C divides 64) that is blk_lane_offset(lane) + y * C + m * 64 * H, so a row store or load
+// needs one shift-and-mask per lane instead of a division per element.
+__device__ __forceinline__ long long blk_lane_offset(int lane, int H, int C) {
+  const unsigned ul = (unsigned)lane, lc = (unsigned)__builtin_ctz((unsigned)C);
+  return (long long)(ul >> lc) * H * C + (ul & (unsigned)(C - 1));
+}
+
 }  // namespace hfx

@@ -1119,6 +1119,7 @@ hfx_plan_desc moments_desc(const Json &j, const TDesc &d) {
   p.height = (int32_t)d.shape[r - 2];
   p.width = (int32_t)d.shape[r - 1];
   p.stride = (int32_t)j.get_int("stride", p.window);
+  p.batch_lanes = 1;  // one window per execute: no second lane
   p.fs_hz = j.get("fs", 37e3);
   p.weight_fs_hz = j.get("weight_fs", 37e3);
   p.lambda_m = f32(j, "lambda");

@@ -162,7 +162,9 @@ hfx_status copy_frames(hfx_server *s, Stream &st, int64_t first, int64_t n) {
 hfx_status build_stream(hfx_server *s, Stream &st) {
   const Geometry &g = st.g;
   hipStream_t cs = s->capture;
-  SRV_TRY(hfx_plan_create(&st.d.plan, cs, &st.plan));
+  hfx_plan_desc pd = st.d.plan;
+  pd.batch_lanes = 1;  // windows run one at a time through hfx_moments: no second lane
+  SRV_TRY(hfx_plan_create(&pd, cs, &st.plan));
   SRV_HIP(hipMalloc((void **)&st.ring, (size_t)g.ring_bytes));
   SRV_HIP(hipMalloc((void **)&st.pool, (size_t)g.pool_bytes));
   SRV_HIP(hipMalloc((void **)&st.outs, (size_t)g.out_bytes));

@@ -25,10 +25,6 @@ constexpr bool is_mom(int k) { return k == BLK_TO_MOM || k == BLK_TO_MOM5; }
 // transfer function, inverse).  The launchers reject a PassArgs::inv that disagrees.
 constexpr bool row_inv(int k) { return k != NAT_TO_BLK; }
 
-__device__ __forceinline__ long long blk_index(int y, int x, int H, int C) {
-  return (long long)(x / C) * H * C + (long long)y * C + (x % C);
-}
-
 template <int E, int NACC> struct Moments {
   float s[NACC][E];
   __device__ __forceinline__ void zero() {
@@ -58,10 +54,12 @@ template <int E, int NACC> struct Moments {
 };
 
 __device__ __forceinline__ void store_moment(const PassArgs &a, int nacc, int y, int x, const float (&val)[5]) {
-  int ys = y, xs = x;
-  if (a.mom.shift) {
-    ys = (y + a.H - a.H / 2) % a.H;
-    xs = (x + a.W - a.W / 2) % a.W;
+  int ys = y, xs = x;  // 0 <= y < H, 0 <= x < W
+  if (a.mom.shift) {   // (y + H - H/2) mod H, without a division
+    ys = y + (a.H - a.H / 2);
+    if (ys >= a.H) ys -= a.H;
+    xs = x + (a.W - a.W / 2);
+    if (xs >= a.W) xs -= a.W;
   }
   const long long o = (long long)ys * a.W + xs;
   const float n_all = (float)a.planes;
@@ -93,6 +91,7 @@ __global__ __launch_bounds__(64 * WROW_WAVES) void k_wrow(PassArgs a) {
   if (y >= a.H) return;
   const int C = a.blk_c;
   const long long plane_elems = (long long)a.H * a.W;
+  const long long boff = blk_lane_offset(lane, a.H, C) + (long long)y * C;  // blocked (y, lane)
   float2 *ws = scr + wave * F::SCR;
 
   auto load = [&](float2 (&v)[E], long long p) {
@@ -112,7 +111,7 @@ __global__ __launch_bounds__(64 * WROW_WAVES) void k_wrow(PassArgs a) {
         }
         v[m] = t;
       } else {
-        v[m] = reinterpret_cast<const float2 *>(a.in)[p * plane_elems + blk_index(y, x, a.H, C)];
+        v[m] = reinterpret_cast<const float2 *>(a.in)[p * plane_elems + boff + (long long)m * 64 * a.H];
       }
     }
   };
@@ -158,7 +157,7 @@ __global__ __launch_bounds__(64 * WROW_WAVES) void k_wrow(PassArgs a) {
       const int x = lane + 64 * m;
       float2 t = v[m];
       if (KIND == NAT_TO_BLK) {
-        reinterpret_cast<float2 *>(a.out)[p * plane_elems + blk_index(y, x, a.H, C)] = t;
+        reinterpret_cast<float2 *>(a.out)[p * plane_elems + boff + (long long)m * 64 * a.H] = t;
       } else {
         const long long o = p * plane_elems + (long long)y * a.W + x;
         if (a.out_mode == OUT_NAT_MAG) {

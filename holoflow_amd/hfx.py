@@ -54,6 +54,7 @@ class MomentConfig:
     split_bin: int = 0
     flatfield: Optional[tuple] = None      # (sigma_y, sigma_x)
     batch_windows: int = 1                 # windows per front launch in moments_batch
+    batch_lanes: int = 0                   # streams moments_batch alternates over (0 = library default)
 
     def to_desc(self) -> _lib.PlanDesc:
         d = _lib.PlanDesc()
@@ -75,6 +76,7 @@ class MomentConfig:
         d.split_bin = self.split_bin
         d.weight_fs_hz = self.weight_fs_hz
         d.batch_windows = self.batch_windows
+        d.batch_lanes = self.batch_lanes
         if self.flatfield is not None:
             d.flatfield_enabled = 1
             d.flatfield_sigma_y, d.flatfield_sigma_x = (float(v) for v in self.flatfield)
@@ -96,7 +98,7 @@ class MomentPlan:
         self.workspace_bytes = info.workspace_bytes
 
     def close(self):
-        if self._h:
+        if self._h and lib is not None:      # module globals are gone at interpreter exit
             lib().hfx_plan_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -188,7 +190,7 @@ class Fresnel:
         return out
 
     def close(self):
-        if self._h:
+        if self._h and lib is not None:      # module globals are gone at interpreter exit
             lib().hfx_fresnel_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -217,7 +219,7 @@ class AngularSpectrum:
         return out
 
     def close(self):
-        if self._h:
+        if self._h and lib is not None:      # module globals are gone at interpreter exit
             lib().hfx_asm_destroy(self._h)
             self._h = C.c_void_p()
 

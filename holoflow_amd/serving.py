@@ -264,7 +264,7 @@ class Server:
         return {k: getattr(i, k) for k, _ in _lib.ServerInfo._fields_}
 
     def close(self):
-        if self._h:
+        if self._h and lib is not None:      # module globals are gone at interpreter exit
             lib().hfx_server_destroy(self._h)
             self._h = C.c_void_p()
 

@@ -149,6 +149,46 @@ def test_multi_window_front_matches_single_windows(gpu, prop, W, nwin):
         assert torch.equal(batch[w], ref), f"window {w}"
 
 
+@pytest.mark.parametrize("prop,W,nwin,prec", [("fresnel", 1024, 1, "fp32"), ("asm", 512, 2, "fp32"),
+                                               ("fresnel", 512, 1, "fp16_split")])
+def test_two_lanes_match_one_lane(gpu, prop, W, nwin, prec):
+    """batch_lanes = 2: window groups alternate between the caller's stream and the plan's own
+    stream (forked from and joined into the caller's).  Every window must equal the one-lane
+    result bit for bit once the caller's stream is synchronised, on a non-default caller stream
+    too; a Flatfield plan keeps one lane (shared scratch images)."""
+    from holoflow_amd import hfx
+
+    T, S, F = 64, 16, 200
+    p = O.PathParams(64, W, T, S, 11, 26, prop, 852e-9, 12e-6, 12e-6, 0.45, fftshift=True)
+    ring = _frames(64, W, F)
+    plans = []
+    for lanes in (1, 2):
+        cfg = _cfg(p, prec)
+        cfg.batch_windows = nwin
+        cfg.batch_lanes = lanes
+        plans.append(hfx.MomentPlan(cfg))
+    one, two = plans
+    assert two.workspace_bytes > one.workspace_bytes
+    first = F - 40
+    ref = one.moments_batch(ring, first, 7)
+    assert torch.equal(two.moments_batch(ring, first, 7), ref)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        out = two.moments_batch(ring, first, 7, stream=side)
+    torch.cuda.current_stream().wait_stream(side)
+    assert torch.equal(out, ref)
+    # one window: no fork; a partial second group: both lanes used once
+    assert torch.equal(two.moments_batch(ring, first, 1)[0], ref[0])
+    assert torch.equal(two.moments_batch(ring, first, nwin + 1), ref[: nwin + 1])
+
+    cfg = _cfg(p, prec)
+    cfg.flatfield = (3.0, 3.0)
+    ff_default = hfx.MomentPlan(cfg)
+    cfg.batch_lanes = 1
+    assert ff_default.workspace_bytes == hfx.MomentPlan(cfg).workspace_bytes
+
+
 @pytest.mark.parametrize("prop,W", [("fresnel", 512), ("asm", 64)])
 def test_stages_compose_to_moments(gpu, prop, W):
     # stage 0 then stage 1 is hfx_moments (fused front kernel at 512, unfused at 64)
