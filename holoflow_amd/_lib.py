@@ -81,6 +81,7 @@ class PlanInfo(C.Structure):
         ("out_height", C.c_int32), ("out_width", C.c_int32),
         ("n_outputs", C.c_int32), ("n_planes", C.c_int32),
         ("workspace_bytes", C.c_int64),
+        ("batch_lanes", C.c_int32),
     ]
 
 

@@ -343,6 +343,7 @@ hfx_status hfx_plan_get_info(const hfx_plan *p, hfx_plan_info *info) {
   info->n_outputs = p->n_out;
   info->n_planes = p->n_planes;
   info->workspace_bytes = (int64_t)(p->arena.bytes + p->f2d.arena.bytes);
+  info->batch_lanes = p->lanes;
   return HFX_OK;
 }
 

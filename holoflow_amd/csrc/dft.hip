@@ -159,7 +159,8 @@ __global__ __launch_bounds__(row_threads<W>()) void k_dft_row(DftArgs a) {
 #pragma unroll
       for (int i = 0; i < TPW; ++i) {
         const int x0 = (wave * TPW + i) * 64 + 4 * (lane & 15);  // first of the lane's 4 pixels
-        float2 L[4];
+        // lens factors; 1 without a lens (the product with (1, 0) is exact, and cheaper than a select)
+        float2 L[4] = {make_float2(1.f, 0.f), make_float2(1.f, 0.f), make_float2(1.f, 0.f), make_float2(1.f, 0.f)};
         if (lens) {
           const float2 ly = a.ly[y];
 #pragma unroll
@@ -190,7 +191,7 @@ __global__ __launch_bounds__(row_threads<W>()) void k_dft_row(DftArgs a) {
 #pragma unroll
               for (int t = 0; t < 4; ++t) {
                 float2 c = side ? make_float2(v[t].x, -v[t].y) : v[t];
-                dst[F::pad(x0 + t)] = lens ? cmul(c, L[t]) : c;
+                dst[F::pad(x0 + t)] = cmul(c, L[t]);
               }
             }
           }
@@ -300,7 +301,8 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
 #pragma unroll
     for (int i = 0; i < TPW; ++i) {
       const int x0 = (rw * TPW + i) * 64 + 4 * (lane & 15);
-      float2 L[4];
+      // lens factors; 1 without a lens (the product with (1, 0) is exact, and cheaper than a select)
+      float2 L[4] = {make_float2(1.f, 0.f), make_float2(1.f, 0.f), make_float2(1.f, 0.f), make_float2(1.f, 0.f)};
       if (lens) {
         const float2 ly = a.ly[y];
 #pragma unroll
@@ -330,7 +332,7 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
               float2 c = side ? make_float2(v[t].x, -v[t].y) : v[t];
-              dst[F::pad(x0 + t)] = lens ? cmul(c, L[t]) : c;
+              dst[F::pad(x0 + t)] = cmul(c, L[t]);
             }
           }
         }

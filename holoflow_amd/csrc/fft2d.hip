@@ -194,7 +194,7 @@ __global__ __launch_bounds__((Geo<N, COLS>::NT)) void k_pass(PassArgs a) {
           float2 v = lds[li];
           if constexpr (MOM) {
             float m = v.x * v.x + v.y * v.y;
-            if (!a.mom.power2) m = sqrtf(m);
+            if (!a.mom.power2) m = mag_sqrt(m);
             acc[0][it] += m;
             acc[1][it] += __fmul_rn(f, m);
             acc[2][it] += __fmul_rn(f2, m);

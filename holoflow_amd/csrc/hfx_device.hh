@@ -153,6 +153,11 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
   return (b & 7) * (n8 >> 3) + (b >> 3);
 }
 
+// |X| for the moment sums: the hardware square root (v_sqrt_f32, within 1 ulp) instead of the
+// correctly rounded sqrtf sequence, which costs about ten more VALU instructions per element in
+// the column pass; the moments are sums over planes held to a relative tolerance, not bits.
+__device__ __forceinline__ float mag_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // Column-blocked layout of an H x W plane with column blocks of C in {1, 2, 4}: element
 // (y, x) sits at (x / C) * H * C + y * C + x % C.  For the wave-owned row x = lane + 64 m
 // (C divides 64) that is blk_lane_offset(lane) + y * C + m * 64 * H, so a row store or load

@@ -39,7 +39,7 @@ template <int E, int NACC> struct Moments {
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       float a = v[m].x * v[m].x + v[m].y * v[m].y;
-      if (!mo.power2) a = sqrtf(a);
+      if (!mo.power2) a = mag_sqrt(a);
       s[0][m] += a;
       s[1][m] += __fmul_rn(f, a);
       s[2][m] += __fmul_rn(f2, a);
@@ -200,10 +200,14 @@ __global__ __launch_bounds__(is_mom(KIND) ? 256 * wcol_subsets(N) : 256) void k_
 
   const int C = a.blk_c;
   const int c = wave & 3, sub = wave >> 2;
-  const int x = blockIdx.x * C + c;
+  // column tiles in contiguous runs per XCD: the moment images are written column-wise, so
+  // the 8 tiles that share an output cache line (32 floats) then share an L2 that merges
+  // their 16-byte pieces before they go out
+  const int bx = xcd_tile(blockIdx.x, gridDim.x);
+  const int x = bx * C + c;
   const bool active = c < C;
   const long long plane_elems = (long long)a.H * a.W;
-  const long long tile = (long long)blockIdx.x * a.H * C;  // blocked offset of the tile
+  const long long tile = (long long)bx * a.H * C;  // blocked offset of the tile
   float2 *ws = scr + wave * F::SCR;
 
   auto load = [&](float2 (&v)[E], long long p) {

@@ -96,6 +96,7 @@ class MomentPlan:
         self.out_shape = (info.n_outputs, info.out_height, info.out_width)
         self.n_planes = info.n_planes
         self.workspace_bytes = info.workspace_bytes
+        self.batch_lanes = info.batch_lanes
 
     def close(self):
         if self._h and lib is not None:      # module globals are gone at interpreter exit

@@ -98,6 +98,7 @@ typedef struct hfx_plan_info {
   int32_t n_outputs;             /* 3, or 5 with a band split */
   int32_t n_planes;              /* propagated planes (nb, or 2*nb for both sidebands) */
   int64_t workspace_bytes;       /* device bytes owned by the plan */
+  int32_t batch_lanes;           /* streams hfx_moments_batch alternates over (1 or 2) */
 } hfx_plan_info;
 
 typedef struct hfx_plan hfx_plan;

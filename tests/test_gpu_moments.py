@@ -168,6 +168,7 @@ def test_two_lanes_match_one_lane(gpu, prop, W, nwin, prec):
         cfg.batch_lanes = lanes
         plans.append(hfx.MomentPlan(cfg))
     one, two = plans
+    assert (one.batch_lanes, two.batch_lanes) == (1, 2)
     assert two.workspace_bytes > one.workspace_bytes
     first = F - 40
     ref = one.moments_batch(ring, first, 7)
@@ -182,9 +183,12 @@ def test_two_lanes_match_one_lane(gpu, prop, W, nwin, prec):
     assert torch.equal(two.moments_batch(ring, first, 1)[0], ref[0])
     assert torch.equal(two.moments_batch(ring, first, nwin + 1), ref[: nwin + 1])
 
+    # library default: two lanes for the pipelined FP32 front only
+    assert hfx.MomentPlan(_cfg(p, prec)).batch_lanes == (2 if prec == "fp32" else 1)
     cfg = _cfg(p, prec)
     cfg.flatfield = (3.0, 3.0)
     ff_default = hfx.MomentPlan(cfg)
+    assert ff_default.batch_lanes == 1
     cfg.batch_lanes = 1
     assert ff_default.workspace_bytes == hfx.MomentPlan(cfg).workspace_bytes
 
