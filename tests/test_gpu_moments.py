@@ -116,6 +116,21 @@ def test_mixed_precision_time_dft_vs_fp64(gpu, precision, tol):
         assert errs["f64"][0] <= 1e-4   # the mixed path's M0 bar
 
 
+@pytest.mark.parametrize("precision,tol", [("fp16_split", 1e-5), ("fp16", 1e-3), ("bf16", 2e-2)])
+@pytest.mark.parametrize("H,W,T,S,kb,ke,prop", [(512, 512, 64, 16, 11, 26, "fresnel"),
+                                                (64, 1024, 32, 8, 6, 13, "asm")])
+def test_fused_16bit_front_vs_fp64(gpu, precision, tol, H, W, T, S, kb, ke, prop):
+    """The fused time-DFT + row-FFT kernel in the 16-bit modes (32-frame MFMA steps, several rows
+    per workgroup at 512 rows) against the FP64 oracle, on a window that wraps around the ring's
+    end."""
+    p = O.PathParams(H, W, T, S, kb, ke, prop, 852e-9, 12e-6, 12e-6, 0.45, fftshift=True)
+    F = T + 2 * S
+    ring = _frames(H, W, F, radius=0.39 * min(H, W))
+    _, errs = _check(p, ring, F - S, precision, against=("f64",), tol=tol)
+    if precision == "fp16_split":
+        assert errs["f64"][0] <= 1e-4
+
+
 def test_batch_matches_single_windows(gpu):
     from holoflow_amd import hfx
 
