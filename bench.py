@@ -271,7 +271,7 @@ def main():
                     help="launches of the dominant kernel timed with HIP events")
     ap.add_argument("--batch-lanes", type=int, default=0,
                     help="streams the batch alternates windows over (0: the library's default, "
-                         "two for the FP32 front; 1 keeps kernel-trace durations unoverlapped)")
+                         "two here; 1 keeps kernel-trace durations unoverlapped)")
     args = ap.parse_args()
 
     if args.gpus != 1:

@@ -277,13 +277,10 @@ hfx_status hfx_plan_create(const hfx_plan_desc *dp, hfx_stream_t stream, hfx_pla
       hfx::dft_front_pipelined(p->W, p->nbc, d.window, p->n_planes, d.precision))
     p->nwin = std::min<int>(d.batch_windows, hfx::HFX_MAX_WIN);
   // batch lanes: a second stream (and plane workspace) lets one window's front overlap the
-  // previous window's back; the Flatfield's scratch images are shared, so it keeps one.
-  // Default: two for the pipelined FP32 front kernel (measured +3 % at 1024^2, T = 64), one
-  // for the 16-bit modes, whose fused kernel measured 6 % slower with two.
+  // previous window's back (measured +15 % at 1024^2, T = 64, in FP32 and the 16-bit modes);
+  // the Flatfield's scratch images are shared, so it keeps one lane.
   if (d.batch_lanes < 0 || d.batch_lanes > 2) return fail(HFX_E_INVALID, "batch_lanes must be 0, 1 or 2");
-  const bool pipelined = p->fused_row && hfx::dft_front_pipelined(p->W, p->nbc, d.window, p->n_planes, d.precision);
-  const int want = d.batch_lanes == 0 ? (pipelined ? 2 : 1) : d.batch_lanes;
-  p->lanes = want == 2 && p->fused_row && !d.flatfield_enabled ? 2 : 1;
+  p->lanes = d.batch_lanes != 1 && p->fused_row && !d.flatfield_enabled ? 2 : 1;
   if (p->lanes == 2) {
     HFX_TRY_HIP(hipStreamCreateWithFlags(&p->aux, hipStreamNonBlocking));
     HFX_TRY_HIP(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));

@@ -85,12 +85,11 @@ typedef struct hfx_plan_desc {
                                 without padding or Filter2D, else 1) */
   int32_t batch_lanes;       /* hfx_moments_batch: HIP streams the windows alternate over, so
                                 that one window's front kernel overlaps the previous window's
-                                back kernel (0 = default: 2 for the pipelined FP32 front kernel,
-                                else 1; 1 = the caller's stream only; 2 = two wherever the time
-                                DFT is fused with the row FFT and no Flatfield runs).  Each lane
-                                holds its own plane workspace; the second lane runs on a stream
-                                the plan owns and joins the caller's stream before the call
-                                returns. */
+                                back kernel (0 or 2 = two wherever the time DFT is fused with
+                                the row FFT and no Flatfield runs, else one; 1 = the caller's
+                                stream only).  Each lane holds its own plane workspace; the
+                                second lane runs on a stream the plan owns and joins the
+                                caller's stream before the call returns. */
 } hfx_plan_desc;
 
 typedef struct hfx_plan_info {

@@ -198,8 +198,8 @@ def test_two_lanes_match_one_lane(gpu, prop, W, nwin, prec):
     assert torch.equal(two.moments_batch(ring, first, 1)[0], ref[0])
     assert torch.equal(two.moments_batch(ring, first, nwin + 1), ref[: nwin + 1])
 
-    # library default: two lanes for the pipelined FP32 front only
-    assert hfx.MomentPlan(_cfg(p, prec)).batch_lanes == (2 if prec == "fp32" else 1)
+    # library default: two lanes wherever the front is fused
+    assert hfx.MomentPlan(_cfg(p, prec)).batch_lanes == 2
     cfg = _cfg(p, prec)
     cfg.flatfield = (3.0, 3.0)
     ff_default = hfx.MomentPlan(cfg)
