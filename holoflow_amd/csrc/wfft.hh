@@ -196,21 +196,29 @@ template <int N> struct WaveFft {
     constexpr bool LAST = NS * R == N;
     constexpr int TOFF = tw_offset(NS);
     float2 u[PER][R];
+    float2 w[PER][R];  // w[q][0] unused
 #pragma unroll
     for (int q = 0; q < PER; ++q)
 #pragma unroll
       for (int r = 0; r < R; ++r) u[q][r] = scr[pad(lane + 64 * q + r * NB)];
+    // all twiddle reads issued together with the data reads, so their latencies overlap
+    // instead of one LDS round trip per pair; the empty asm keeps the compiler from sinking
+    // each read down to its use
+#pragma unroll
+    for (int q = 0; q < PER; ++q)
+#pragma unroll
+      for (int r = 1; r < R; ++r) w[q][r] = tw[TOFF + (r - 1) * NS + (lane + 64 * q) % NS];
+#pragma unroll
+    for (int q = 0; q < PER; ++q)
+#pragma unroll
+      for (int r = 1; r < R; ++r) asm volatile("" : "+v"(w[q][r].x), "+v"(w[q][r].y));
     __builtin_amdgcn_wave_barrier();
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-      const int j = lane + 64 * q;
-      const int k = j % NS;
 #pragma unroll
-      for (int r = 1; r < R; ++r) {
+      for (int r = 1; r < R; ++r)
         // the table holds the forward twiddles exp(-2 pi i m/N); the inverse conjugates them
-        const float2 w = tw[TOFF + (r - 1) * NS + k];
-        u[q][r] = S < 0 ? cmul(u[q][r], w) : cmulc(u[q][r], w);
-      }
+        u[q][r] = S < 0 ? cmul(u[q][r], w[q][r]) : cmulc(u[q][r], w[q][r]);
       RegDft<R, S>::run(u[q]);
     }
     if constexpr (LAST) {
