@@ -178,8 +178,10 @@ __global__ __launch_bounds__(64 * WROW_WAVES) void k_wrow(PassArgs a) {
 // values) and 5*E moment sums in registers.  Three subsets (12 waves, 3 per SIMD) without
 // a register prefetch measured faster at 1024 rows than two subsets with the next plane
 // prefetched into registers; 2048-row columns leave room for one subset only.
-constexpr int wcol_subsets(int n) { return n >= 2048 ? 1 : 3; }
-constexpr bool wcol_prefetch(int n) { return n >= 2048 ? false : wcol_subsets(n) < 3; }
+// The five-sum kind (band split) keeps 5*E sums per lane: two subsets at 1024 rows (8 waves,
+// 256 registers each) instead of three, which spilled.
+constexpr int wcol_subsets(int n, int kind) { return n >= 2048 ? 1 : kind == BLK_TO_MOM5 ? 2 : 3; }
+constexpr bool wcol_prefetch(int n, int kind) { return n >= 2048 ? false : wcol_subsets(n, kind) < 3; }
 // Planes per block of the plane-parallel column kinds.  The 2048-row middle pass (forward
 // FFT, table, inverse FFT) runs one wave per SIMD (registers), so a block walks 4 planes with
 // the next plane's column prefetched into registers (measured 13 % faster at 2048^2); the
@@ -187,10 +189,10 @@ constexpr bool wcol_prefetch(int n) { return n >= 2048 ? false : wcol_subsets(n)
 constexpr int wcol_ppb(int n, int kind) { return n >= 2048 && kind == BLK_MID ? 4 : 1; }
 
 template <int N, int KIND>
-__global__ __launch_bounds__(is_mom(KIND) ? 256 * wcol_subsets(N) : 256) void k_wcol(PassArgs a) {
+__global__ __launch_bounds__(is_mom(KIND) ? 256 * wcol_subsets(N, KIND) : 256) void k_wcol(PassArgs a) {
   using F = WaveFft<N>;
   constexpr int E = F::E;
-  constexpr int S = is_mom(KIND) ? wcol_subsets(N) : 1;
+  constexpr int S = is_mom(KIND) ? wcol_subsets(N, KIND) : 1;
   constexpr int WAVES = 4 * S;
   __shared__ float2 scr[WAVES * F::SCR];
   __shared__ float2 twl[F::TW];
@@ -220,7 +222,7 @@ __global__ __launch_bounds__(is_mom(KIND) ? 256 * wcol_subsets(N) : 256) void k_
     constexpr int NACC = KIND == BLK_TO_MOM5 ? 5 : 3;
     Moments<E, NACC> acc;
     acc.zero();
-    if (active && wcol_prefetch(N)) {
+    if (active && wcol_prefetch(N, KIND)) {
       float2 v[E], nxt[E];
       if (sub < a.planes) load(nxt, sub);
       for (int p = sub; p < a.planes; p += S) {
@@ -336,7 +338,7 @@ template <int N, int KIND> static int launch_wcol_n(const PassArgs &a, hipStream
   if (a.inv != 0) return -5;
   const unsigned gx = (unsigned)(a.W / a.blk_c);
   const unsigned gy = is_mom(KIND) ? 1u : (unsigned)((a.planes + wcol_ppb(N, KIND) - 1) / wcol_ppb(N, KIND));
-  const unsigned nt = is_mom(KIND) ? 256 * wcol_subsets(N) : 256;
+  const unsigned nt = is_mom(KIND) ? 256 * wcol_subsets(N, KIND) : 256;
   hipLaunchKernelGGL((k_wcol<N, KIND>), dim3(gx, gy), dim3(nt), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -131,6 +131,14 @@ def test_fused_16bit_front_vs_fp64(gpu, precision, tol, H, W, T, S, kb, ke, prop
         assert errs["f64"][0] <= 1e-4
 
 
+def test_fresnel_band_split_1024_rows(gpu):
+    """Band-split moments (five sums) out of the 1024-row column pass, against both oracles."""
+    p = O.PathParams(1024, 256, 32, 8, 4, 12, "fresnel", 852e-9, 12e-6, 12e-6, 0.45, fftshift=True,
+                     split_bin=8)
+    out, _ = _check(p, _frames(1024, 256, 40, radius=100), 8)
+    assert out.shape == (5, 1024, 256)
+
+
 def test_batch_matches_single_windows(gpu):
     from holoflow_amd import hfx
 
