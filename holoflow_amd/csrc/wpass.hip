@@ -33,13 +33,13 @@ template <int E, int NACC> struct Moments {
 #pragma unroll
       for (int m = 0; m < E; ++m) s[k][m] = 0.f;
   }
-  __device__ __forceinline__ void add(const float2 (&v)[E], const MomentArgs &mo, int plane) {
-    const float f = mo.f[plane], f2 = mo.f2[plane];
-    const bool lo = plane < mo.split;
+  // f, f2: the plane's M1/M2 weights, read by the caller at the top of its plane iteration
+  // (wave-uniform scalar loads) so that their latency is not paid after the transform
+  __device__ __forceinline__ void add(const float2 (&v)[E], float f, float f2, bool lo, bool power2) {
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       float a = v[m].x * v[m].x + v[m].y * v[m].y;
-      if (!mo.power2) a = mag_sqrt(a);
+      if (!power2) a = mag_sqrt(a);
       s[0][m] += a;
       s[1][m] += __fmul_rn(f, a);
       s[2][m] += __fmul_rn(f2, a);
@@ -124,18 +124,20 @@ __global__ __launch_bounds__(64 * WROW_WAVES) void k_wrow(PassArgs a) {
       float2 v[E], nxt[E];
       load(nxt, 0);
       for (int p = 0; p < a.planes; ++p) {
+        const float f = a.mom.f[p], f2 = a.mom.f2[p];
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = nxt[m];
         if (p + 1 < a.planes) load(nxt, p + 1);
         F::template run<row_inv(KIND)>(v, ws, twl, lane);
-        acc.add(v, a.mom, p);
+        acc.add(v, f, f2, p < a.mom.split, a.mom.power2);
       }
     } else {  // 2048-point rows: no register room for a prefetch next to 5*E sums
       for (int p = 0; p < a.planes; ++p) {
+        const float f = a.mom.f[p], f2 = a.mom.f2[p];
         float2 v[E];
         load(v, p);
         F::template run<row_inv(KIND)>(v, ws, twl, lane);
-        acc.add(v, a.mom, p);
+        acc.add(v, f, f2, p < a.mom.split, a.mom.power2);
       }
     }
 #pragma unroll
@@ -201,7 +203,8 @@ __global__ __launch_bounds__(is_mom(KIND) ? 256 * wcol_subsets(N, KIND) : 256) v
   __syncthreads();
 
   const int C = a.blk_c;
-  const int c = wave & 3, sub = wave >> 2;
+  // wave-uniform to the compiler too: plane indices, weights and tile offsets become scalars
+  const int c = __builtin_amdgcn_readfirstlane(wave & 3), sub = __builtin_amdgcn_readfirstlane(wave >> 2);
   // column tiles in contiguous runs per XCD: the moment images are written column-wise, so
   // the 8 tiles that share an output cache line (32 floats) then share an L2 that merges
   // their 16-byte pieces before they go out
@@ -226,18 +229,20 @@ __global__ __launch_bounds__(is_mom(KIND) ? 256 * wcol_subsets(N, KIND) : 256) v
       float2 v[E], nxt[E];
       if (sub < a.planes) load(nxt, sub);
       for (int p = sub; p < a.planes; p += S) {
+        const float f = a.mom.f[p], f2 = a.mom.f2[p];
 #pragma unroll
         for (int m = 0; m < E; ++m) v[m] = nxt[m];
         if (p + S < a.planes) load(nxt, p + S);
         F::template run<false>(v, ws, twl, lane);
-        acc.add(v, a.mom, p);
+        acc.add(v, f, f2, p < a.mom.split, a.mom.power2);
       }
     } else if (active) {  // more waves per CU instead of a register prefetch
       for (int p = sub; p < a.planes; p += S) {
+        const float f = a.mom.f[p], f2 = a.mom.f2[p];
         float2 v[E];
         load(v, p);
         F::template run<false>(v, ws, twl, lane);
-        acc.add(v, a.mom, p);
+        acc.add(v, f, f2, p < a.mom.split, a.mom.power2);
       }
     }
     // fixed-order combination of the plane subsets: sub 0 += sub 1, sub 2, ... (LDS reused)
