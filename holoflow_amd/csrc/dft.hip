@@ -15,6 +15,8 @@
 // Plane q of the output: q < nb -> bin kb+q; q >= nb (two-sided path) -> conj of computed
 // bin 2*nb - q, i.e. X[T - (kb + 2*nb - q)] of a real input (fft.cu:163-194 +
 // graph_builder.cc:270-287).
+#include <type_traits>
+
 #include "dft_core.hh"
 #include "hfx_kernels.hh"
 #include "wfft.hh"
@@ -264,11 +266,17 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
   __shared__ float2 twl[F::TW];
   extern __shared__ float4 wmat[];     // operand matrix of the MFMA loop
 
+  // one 16-bin tile (NBT == 2: up to 16 bins as 32 interleaved cos/-sin rows): the folded
+  // contraction of DftStreamFold, half the MFMAs
+  constexpr bool FOLD = NBT == 2;
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const bool producer = wave < NPW;    // wave-uniform role
   const int rw = producer ? wave : wave - NPW;
-  dft_stage_matrix<NBT, MODE>(wmat, a.wm_hi, a.wm_lo, a.r.T, tid, NT);
+  if constexpr (FOLD)
+    stage_fold_matrix<T>(reinterpret_cast<float *>(wmat), reinterpret_cast<const float *>(a.wm_hi), tid, NT);
+  else
+    dft_stage_matrix<NBT, MODE>(wmat, a.wm_hi, a.wm_lo, a.r.T, tid, NT);
   F::build_twiddles(twl, a.tw, tid, NT);
   __syncthreads();
 
@@ -279,8 +287,9 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
   const int nrw = y0 < a.H ? (a.H - 1 - y0) / G + 1 : 0;  // rows of one window
   const int nr = nrw * a.nwin;                             // rows of all windows, window-major
 
-  f4 acc[TPW][NBT][4];
-  DftStream<NBT, TPW, T, front_ring(T)> stream;
+  f4 acc[TPW][NBT][4];        // interleaved rows (!FOLD)
+  f4 fre[TPW][4], fim[TPW][4];  // Re and Im of bins 4 g + r (FOLD)
+  std::conditional_t<FOLD, DftStreamFold<TPW, T, 4>, DftStream<NBT, TPW, T, front_ring(T)>> stream;
   auto px0 = [&](int y) { return (long long)y * W + rw * 64 * TPW; };
   auto row_y = [&](int i) { return y0 + (nrw ? i % nrw : 0) * G; };
   auto ring_of = [&](int i) {
@@ -293,11 +302,37 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
   auto dft_row = [&](int i, int lane) {
     const bool has_next = i + 1 < nr;
     const int j = has_next ? i + 1 : i;
-    stream.row(ring_of(i), ring_of(j), reinterpret_cast<const float *>(wmat), px0(row_y(i)), px0(row_y(j)), has_next,
-               lane, acc);
+    if constexpr (FOLD)
+      stream.row(ring_of(i), ring_of(j), wmat, px0(row_y(i)), px0(row_y(j)), has_next, lane, fre, fim);
+    else
+      stream.row(ring_of(i), ring_of(j), reinterpret_cast<const float *>(wmat), px0(row_y(i)), px0(row_y(j)),
+                 has_next, lane, acc);
   };
   // producers: MFMA registers -> padded LDS rows (lens applied for Fresnel)
   auto scatter = [&](int y, int lane) {
+    // band bin `bin` of 4 pixels from x0 into the LDS plane rows (and its conjugate plane on
+    // the two-sided path), lens applied
+    auto put = [&](int x0, int bin, const float2 (&v)[4], const float2 (&L)[4]) {
+#pragma unroll
+      for (int side = 0; side < 2; ++side) {
+        int q;
+        if (side == 0) {
+          if (bin >= a.nb) continue;
+          q = bin;
+        } else {
+          if (!a.both) continue;
+          const int j = a.nb - bin;
+          if (j < 0 || j >= a.nb) continue;
+          q = a.nb + j;
+        }
+        float2 *dst = scr + q * F::SCR;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          float2 c = side ? make_float2(v[t].x, -v[t].y) : v[t];
+          dst[F::pad(x0 + t)] = cmul(c, L[t]);
+        }
+      }
+    };
 #pragma unroll
     for (int i = 0; i < TPW; ++i) {
       const int x0 = (rw * TPW + i) * 64 + 4 * (lane & 15);
@@ -308,37 +343,36 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) L[t] = cmul(a.lx[x0 + t], ly);
       }
+      if constexpr (FOLD) {
 #pragma unroll
-      for (int bt = 0; bt < NBT; ++bt) {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int bin = bt * 8 + 2 * (lane >> 4) + h;
+        for (int r = 0; r < 4; ++r) {
+          const int bin = 4 * (lane >> 4) + r;
           if (bin >= a.nbc) continue;
           float2 v[4];
-          dft_bin_values(a, acc[i][bt], h, bin, v);
 #pragma unroll
-          for (int side = 0; side < 2; ++side) {
-            int q;
-            if (side == 0) {
-              if (bin >= a.nb) continue;
-              q = bin;
-            } else {
-              if (!a.both) continue;
-              const int j = a.nb - bin;
-              if (j < 0 || j >= a.nb) continue;
-              q = a.nb + j;
-            }
-            float2 *dst = scr + q * F::SCR;
+          for (int t = 0; t < 4; ++t) v[t] = make_float2(fre[i][t][r], fim[i][t][r]);
+          if (a.kb + bin == 0) {  // the centring offset restored on bin 0
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-              float2 c = side ? make_float2(v[t].x, -v[t].y) : v[t];
-              dst[F::pad(x0 + t)] = cmul(c, L[t]);
-            }
+            for (int t = 0; t < 4; ++t) v[t].x += 2048.0f * (float)a.r.T;
+          }
+          put(x0, bin, v, L);
+        }
+      } else {
+#pragma unroll
+        for (int bt = 0; bt < NBT; ++bt) {
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const int bin = bt * 8 + 2 * (lane >> 4) + h;
+            if (bin >= a.nbc) continue;
+            float2 v[4];
+            dft_bin_values(a, acc[i][bt], h, bin, v);
+            put(x0, bin, v, L);
           }
         }
       }
     }
   };
+
   // consumers: one wave per plane row: forward row FFT, then the column-blocked store
   auto transform = [&](int i, int lane) {
     const int y = row_y(i);

@@ -317,4 +317,113 @@ template <int NBT, int TPW, int T, int K> struct DftStream {
   }
 };
 
+// FP32 time DFT of one 16-bin tile with the real-input symmetry folded in (T even):
+//   Re X_k = x_0 + (-1)^k x_{T/2} + sum_{p=1}^{T/2-1} (x_p + x_{T-p}) cos(2 pi k p / T)
+//   Im X_k =                      - sum_{p=1}^{T/2-1} (x_p - x_{T-p}) sin(2 pi k p / T)
+// (cos and sin are even and odd about T/2), which halves the MFMAs of the interleaved
+// (cos, -sin) 32-row contraction: per 16-pixel tile 2 (T/8) + 1 MFMAs instead of T/2.
+// Step s covers the pairs p = 4 s + g + 1 (g = lane >> 4, the MFMA's K index), loading frames
+// p and T - p; the last step's p = T/2 pairs the middle frame with itself under zero weights,
+// and x_0, x_{T/2} enter through one more MFMA.  The operand matrix (stage_fold_matrix)
+// holds a cos block and a sin block of 16 rows x PC floats and a 16 x 4 block for x_0, x_{T/2}.
+// Results: ac[i][t][r] = Re X and as[i][t][r] = Im X of bin 4 g + r for pixel
+// p0w + 64 i + 4 (lane & 15) + t.  The ring of K steps (two frames per tile each) runs on into
+// the next row as in DftStream.
+template <int TPW, int T, int K> struct DftStreamFold {
+  static constexpr int STEPS = T / 8;
+  static constexpr int PC = T / 2 + 4;  // pitch: 16 rows x 4 K-slots hit 64 distinct banks
+  static_assert(STEPS % K == 0, "ring depth must divide the steps of a row");
+  uint2 ring[K][TPW][2];
+
+  __device__ __forceinline__ void load(const DftRing &r, long long p0w, int lane, int s, int slot) {
+    const int p = 4 * s + (lane >> 4) + 1;
+    const long long f1 = ring_frame(r, p), f2 = ring_frame(r, T - p);
+    const long long p0 = p0w + 4 * (lane & 15);
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) {
+      ring[slot][i][0] = dft_load4(r, f1, p0 + 64 * i);
+      ring[slot][i][1] = dft_load4(r, f2, p0 + 64 * i);
+    }
+  }
+  __device__ __forceinline__ void start(const DftRing &r, long long p0w, int lane) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) load(r, p0w, lane, k, k);
+  }
+  __device__ __forceinline__ void row(const DftRing &r, const DftRing &r_next, const void *lds, long long p0w,
+                                      long long p0w_next, bool has_next, int lane, f4 (&ac)[TPW][4],
+                                      f4 (&as)[TPW][4]) {
+    const float *lc = reinterpret_cast<const float *>(lds), *ls = lc + 16 * PC, *lx = ls + 16 * PC;
+    const int g = lane >> 4, rw = lane & 15;
+#pragma unroll
+    for (int i = 0; i < TPW; ++i)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        ac[i][t] = f4{0.f, 0.f, 0.f, 0.f};
+        as[i][t] = f4{0.f, 0.f, 0.f, 0.f};
+      }
+    // x_0 (g = 0) and x_{T/2} (g >= 1; zero weights for g >= 2), used after the pair steps
+    uint2 xe[TPW];
+    {
+      const long long fe = ring_frame(r, g == 0 ? 0 : T / 2);
+      const long long p0 = p0w + 4 * (lane & 15);
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) xe[i] = dft_load4(r, fe, p0 + 64 * i);
+    }
+#pragma unroll
+    for (int s = 0; s < STEPS; ++s) {
+      const int slot = s % K;
+      float sa[TPW][4], sb[TPW][4];
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) {
+        float x1[4], x2[4];
+        unpack4(ring[slot][i][0], x1);
+        unpack4(ring[slot][i][1], x2);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          sa[i][t] = x1[t] + x2[t];  // exact: centred 12-bit samples
+          sb[i][t] = x1[t] - x2[t];
+        }
+      }
+      if (s + K < STEPS)
+        load(r, p0w, lane, s + K, slot);
+      else if (has_next)
+        load(r_next, p0w_next, lane, s + K - STEPS, slot);
+      const float wc = lc[rw * PC + 4 * s + g], wsn = ls[rw * PC + 4 * s + g];
+#pragma unroll
+      for (int i = 0; i < TPW; ++i)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          ac[i][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(wc, sa[i][t], ac[i][t], 0, 0, 0);
+          as[i][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(wsn, sb[i][t], as[i][t], 0, 0, 0);
+        }
+    }
+    const float wx = lx[rw * 4 + g];
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) {
+      float x[4];
+      unpack4(xe[i], x);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) ac[i][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(wx, x[t], ac[i][t], 0, 0, 0);
+    }
+  }
+};
+
+// Stages the folded operand matrix of DftStreamFold from the interleaved (cos, -sin) rows
+// wm[2b][n], wm[2b+1][n] (b < 16, n < T): cos block C[b][p-1] and sin block S[b][p-1] for the
+// pairs p = 1 .. T/2-1 (zero beyond), then X[b] = (cos 0, cos(pi k), 0, 0) for x_0, x_{T/2}.
+template <int T>
+__device__ __forceinline__ void stage_fold_matrix(float *l, const float *wm, int tid, int nthreads) {
+  constexpr int PC = T / 2 + 4;
+  for (int i = tid; i < 16 * PC; i += nthreads) {
+    const int b = i / PC, j = i % PC;
+    const bool in = j < T / 2 - 1;
+    l[i] = in ? wm[(2 * b) * T + j + 1] : 0.f;
+    l[16 * PC + i] = in ? wm[(2 * b + 1) * T + j + 1] : 0.f;
+  }
+  for (int i = tid; i < 64; i += nthreads) {
+    const int b = i / 4, k = i % 4;
+    l[32 * PC + i] = k == 0 ? wm[(2 * b) * T] : k == 1 ? wm[(2 * b) * T + T / 2] : 0.f;
+  }
+}
+
 }  // namespace hfx

@@ -131,6 +131,20 @@ def test_fused_16bit_front_vs_fp64(gpu, precision, tol, H, W, T, S, kb, ke, prop
         assert errs["f64"][0] <= 1e-4
 
 
+@pytest.mark.parametrize("W,T,kb,ke,prop,both", [(512, 32, 0, 12, "fresnel", False),
+                                                 (768, 64, 5, 17, "asm", False),
+                                                 (512, 128, 20, 35, "fresnel", True),
+                                                 (1024, 64, 11, 26, "fresnel", True)])
+def test_folded_front_vs_oracles(gpu, W, T, kb, ke, prop, both):
+    """The FP32 front kernel folds the real-input symmetry of the time DFT (x_p + x_{T-p} on the
+    cosine rows, x_p - x_{T-p} on the sine rows, x_0 and x_{T/2} apart) for 9..16 bins: window
+    lengths 32/64/128, the DC bin in the band (centring offset), both sidebands (16 bins)."""
+    p = O.PathParams(64, W, T, T // 4, kb, ke, prop, 852e-9, 12e-6, 12e-6, 0.4, fftshift=True,
+                     both_sidebands=both)
+    F = T + T // 2
+    _check(p, _frames(64, W, F), F - T // 4)
+
+
 def test_fresnel_band_split_1024_rows(gpu):
     """Band-split moments (five sums) out of the 1024-row column pass, against both oracles."""
     p = O.PathParams(1024, 256, 32, 8, 4, 12, "fresnel", 852e-9, 12e-6, 12e-6, 0.45, fftshift=True,
