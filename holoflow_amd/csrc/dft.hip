@@ -51,6 +51,34 @@ __device__ __forceinline__ void dft_bin_values(const DftArgs &a, const f4 (&acc)
 }
 
 // ----------------------------------------------------------------------- stand-alone
+// Band bin `bin` of the 4 pixels from p0 into its natural-layout plane (and the conjugate
+// plane of the two-sided path), times the lens when there is one.
+__device__ __forceinline__ void dft_store_bin(const DftArgs &a, long long p0, int bin, const float2 (&v)[4],
+                                              const float2 (&L)[4], bool lens) {
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+    int q;
+    if (side == 0) {
+      if (bin >= a.nb) continue;
+      q = bin;
+    } else {
+      if (!a.both) continue;
+      const int j = a.nb - bin;
+      if (j < 0 || j >= a.nb) continue;
+      q = a.nb + j;
+    }
+    float2 o[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      float2 c = side ? make_float2(v[t].x, -v[t].y) : v[t];
+      o[t] = lens ? cmul(c, L[t]) : c;
+    }
+    float4 *dst = reinterpret_cast<float4 *>(a.planes + (long long)q * a.r.npix + p0);
+    dst[0] = make_float4(o[0].x, o[0].y, o[1].x, o[1].y);
+    dst[1] = make_float4(o[2].x, o[2].y, o[3].x, o[3].y);
+  }
+}
+
 template <int NBT, int MODE>
 __global__ __launch_bounds__(256) void k_dft(DftArgs a) {
   extern __shared__ float4 lds_dyn[];
@@ -82,28 +110,49 @@ __global__ __launch_bounds__(256) void k_dft(DftArgs a) {
       if (bin >= a.nbc) continue;
       float2 v[4];
       dft_bin_values(a, acc[bt], h, bin, v);
+      dft_store_bin(a, p0, bin, v, L, lens);
+    }
+  }
+}
+
+// FP32 stand-alone time DFT with the real-input symmetry folded (dft_fold_tile): NBF tiles of
+// 16 bins, T % 32 == 0.  Same outputs as k_dft<NBT, 0> up to FP32 rounding, with
+// 2 (T/8) + 1 MFMAs per 16 bins and 16 pixels instead of T/2.
+template <int NBF>
+__global__ __launch_bounds__(256) void k_dft_fold(DftArgs a) {
+  extern __shared__ float4 lds_dyn[];
+  float *lf = reinterpret_cast<float *>(lds_dyn);
+  stage_fold_matrix_rt(lf, reinterpret_cast<const float *>(a.wm_hi), a.r.T, NBF, a.nbc, threadIdx.x, 256);
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const long long p0w = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+  f4 ac[NBF][4], as[NBF][4];
+  dft_fold_tile<NBF>(a.r, lf, p0w, lane, ac, as);
+
+  const long long p0 = p0w + 4 * (lane & 15);
+  if (p0 >= a.r.npix) return;
+  float2 L[4];
+  const bool lens = a.lx != nullptr;
+  if (lens) {
+    const int y = (int)(p0 / a.W), x = (int)(p0 % a.W);  // W % 4 == 0: same row
+    const float2 ly = a.ly[y];
 #pragma unroll
-      for (int side = 0; side < 2; ++side) {
-        int q;
-        if (side == 0) {
-          if (bin >= a.nb) continue;
-          q = bin;
-        } else {
-          if (!a.both) continue;
-          const int j = a.nb - bin;
-          if (j < 0 || j >= a.nb) continue;
-          q = a.nb + j;
-        }
-        float2 o[4];
+    for (int t = 0; t < 4; ++t) L[t] = cmul(a.lx[x + t], ly);
+  }
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          float2 c = side ? make_float2(v[t].x, -v[t].y) : v[t];
-          o[t] = lens ? cmul(c, L[t]) : c;
-        }
-        float4 *dst = reinterpret_cast<float4 *>(a.planes + (long long)q * a.r.npix + p0);
-        dst[0] = make_float4(o[0].x, o[0].y, o[1].x, o[1].y);
-        dst[1] = make_float4(o[2].x, o[2].y, o[3].x, o[3].y);
+  for (int f = 0; f < NBF; ++f) {
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      const int bin = 16 * f + 4 * (lane >> 4) + rr;
+      if (bin >= a.nbc) continue;
+      float2 v[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) v[t] = make_float2(ac[f][t][rr], as[f][t][rr]);
+      if (a.kb + bin == 0) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) v[t].x += 2048.0f * (float)a.r.T;
       }
+      dft_store_bin(a, p0, bin, v, L, lens);
     }
   }
 }
@@ -468,8 +517,24 @@ template <int NBT> static int launch_plain_mode(const DftArgs &a, int mode, hipS
   return -2;
 }
 
+template <int NBF> static int launch_fold(const DftArgs &a, hipStream_t st) {
+  const long long waves = (a.r.npix + 63) / 64;
+  hipLaunchKernelGGL((k_dft_fold<NBF>), dim3((unsigned)((waves + 3) / 4)), dim3(256),
+                     dft_fold_lds_bytes(NBF, a.r.T), st, a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int launch_time_dft(const DftLaunch &L, hipStream_t st) {
   const DftArgs a = make_args(L);
+  // FP32 with more than one 16-row tile of interleaved rows: the folded contraction
+  if (L.precision == 0 && L.T % 32 == 0 && L.nbc > 8 && L.nbc <= 64) {
+    switch ((L.nbc + 15) / 16) {
+      case 1: return launch_fold<1>(a, st);
+      case 2: return launch_fold<2>(a, st);
+      case 3: return launch_fold<3>(a, st);
+      case 4: return launch_fold<4>(a, st);
+    }
+  }
   switch ((2 * L.nbc + 15) / 16) {
     case 1: return launch_plain_mode<1>(a, L.precision, st);
     case 2: return launch_plain_mode<2>(a, L.precision, st);

@@ -426,4 +426,87 @@ __device__ __forceinline__ void stage_fold_matrix(float *l, const float *wm, int
   }
 }
 
+// Runtime-T form of the folded FP32 time DFT (see DftStreamFold) for NBF 16-bin tiles and one
+// 64-pixel tile per wave (stand-alone time DFT): steps of four frame pairs in batches of K,
+// T % (8 K) == 0.  Operand matrix from stage_fold_matrix_rt with pitch T/2 + 4.
+template <int NBF, int K = 4>
+__device__ __forceinline__ void dft_fold_tile(const DftRing &r, const float *lds, long long p0w, int lane,
+                                              f4 (&ac)[NBF][4], f4 (&as)[NBF][4]) {
+  const int T = r.T, steps = T / 8, PC = T / 2 + 4;
+  const float *lc = lds, *ls = lc + 16 * NBF * PC, *lx = ls + 16 * NBF * PC;
+  const int g = lane >> 4, rw = lane & 15;
+  const long long p0 = p0w + 4 * (lane & 15);
+#pragma unroll
+  for (int f = 0; f < NBF; ++f)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      ac[f][t] = f4{0.f, 0.f, 0.f, 0.f};
+      as[f][t] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+  const uint2 xe = dft_load4(r, ring_frame(r, g == 0 ? 0 : T / 2), p0);  // x_0, x_{T/2}
+  uint2 ring[K][2];
+  auto load = [&](int st, int k) {
+    const int p = 4 * st + g + 1;
+    ring[k][0] = dft_load4(r, ring_frame(r, p), p0);
+    ring[k][1] = dft_load4(r, ring_frame(r, T - p), p0);
+  };
+#pragma unroll
+  for (int k = 0; k < K; ++k) load(k, k);
+  for (int s0 = 0; s0 < steps; s0 += K) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int st = s0 + k;
+      float x1[4], x2[4], sa[4], sb[4];
+      unpack4(ring[k][0], x1);
+      unpack4(ring[k][1], x2);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        sa[t] = x1[t] + x2[t];
+        sb[t] = x1[t] - x2[t];
+      }
+      if (st + K < steps) load(st + K, k);
+#pragma unroll
+      for (int f = 0; f < NBF; ++f) {
+        const float wc = lc[(16 * f + rw) * PC + 4 * st + g], wsn = ls[(16 * f + rw) * PC + 4 * st + g];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          ac[f][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(wc, sa[t], ac[f][t], 0, 0, 0);
+          as[f][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(wsn, sb[t], as[f][t], 0, 0, 0);
+        }
+      }
+    }
+  }
+  float x[4];
+  unpack4(xe, x);
+#pragma unroll
+  for (int f = 0; f < NBF; ++f) {
+    const float wx = lx[(16 * f + rw) * 4 + g];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) ac[f][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(wx, x[t], ac[f][t], 0, 0, 0);
+  }
+}
+
+// Operand matrix of dft_fold_tile from the interleaved rows wm[2b][n], wm[2b+1][n] (b < nbc;
+// rows of bins >= nbc are zero): nbf tiles of 16 bins.
+__device__ __forceinline__ void stage_fold_matrix_rt(float *l, const float *wm, int T, int nbf, int nbc, int tid,
+                                                     int nthreads) {
+  const int PC = T / 2 + 4, rows = 16 * nbf;
+  for (int i = tid; i < rows * PC; i += nthreads) {
+    const int b = i / PC, j = i % PC;
+    const bool in = b < nbc && j < T / 2 - 1;
+    l[i] = in ? wm[(2 * b) * T + j + 1] : 0.f;
+    l[rows * PC + i] = in ? wm[(2 * b + 1) * T + j + 1] : 0.f;
+  }
+  for (int i = tid; i < rows * 4; i += nthreads) {
+    const int b = i / 4, k = i % 4;
+    float w = 0.f;
+    if (b < nbc) w = k == 0 ? wm[(2 * b) * T] : k == 1 ? wm[(2 * b) * T + T / 2] : 0.f;
+    l[2 * rows * PC + i] = w;
+  }
+}
+
+__host__ __device__ constexpr size_t dft_fold_lds_bytes(int nbf, int T) {
+  return (size_t)nbf * (32 * (T / 2 + 4) + 64) * 4;
+}
+
 }  // namespace hfx

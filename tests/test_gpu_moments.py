@@ -145,6 +145,18 @@ def test_folded_front_vs_oracles(gpu, W, T, kb, ke, prop, both):
     _check(p, _frames(64, W, F), F - T // 4)
 
 
+@pytest.mark.parametrize("T,kb,ke,prop,both", [(64, 1, 30, "asm", False),    # 29 bins: two tiles
+                                               (32, 0, 16, "fresnel", True),  # 17 bins with DC: two
+                                               (128, 10, 50, "fresnel", False)])  # 40 bins: three
+def test_folded_standalone_dft_vs_oracles(gpu, T, kb, ke, prop, both):
+    """The stand-alone FP32 time DFT (widths without the fused kernel) folds the real-input
+    symmetry over several 16-bin tiles."""
+    p = O.PathParams(48, 64, T, T // 4, kb, ke, prop, 852e-9, 12e-6, 12e-6, 0.35, fftshift=True,
+                     both_sidebands=both)
+    F = T + T // 2
+    _check(p, _frames(48, 64, F), F - T // 4)
+
+
 def test_fresnel_band_split_1024_rows(gpu):
     """Band-split moments (five sums) out of the 1024-row column pass, against both oracles."""
     p = O.PathParams(1024, 256, 32, 8, 4, 12, "fresnel", 852e-9, 12e-6, 12e-6, 0.45, fftshift=True,
