@@ -323,7 +323,8 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
   const bool producer = wave < NPW;    // wave-uniform role
   const int rw = producer ? wave : wave - NPW;
   if constexpr (FOLD)
-    stage_fold_matrix<T>(reinterpret_cast<float *>(wmat), reinterpret_cast<const float *>(a.wm_hi), tid, NT);
+    stage_fold_matrix_rt(reinterpret_cast<float *>(wmat), reinterpret_cast<const float *>(a.wm_hi), T, 1, a.nbc,
+                         tid, NT);
   else
     dft_stage_matrix<NBT, MODE>(wmat, a.wm_hi, a.wm_lo, a.r.T, tid, NT);
   F::build_twiddles(twl, a.tw, tid, NT);

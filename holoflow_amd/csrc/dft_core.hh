@@ -324,8 +324,9 @@ template <int NBT, int TPW, int T, int K> struct DftStream {
 // (cos, -sin) 32-row contraction: per 16-pixel tile 2 (T/8) + 1 MFMAs instead of T/2.
 // Step s covers the pairs p = 4 s + g + 1 (g = lane >> 4, the MFMA's K index), loading frames
 // p and T - p; the last step's p = T/2 pairs the middle frame with itself under zero weights,
-// and x_0, x_{T/2} enter through one more MFMA.  The operand matrix (stage_fold_matrix)
-// holds a cos block and a sin block of 16 rows x PC floats and a 16 x 4 block for x_0, x_{T/2}.
+// and x_0, x_{T/2} enter through one more MFMA.  The operand matrix (stage_fold_matrix_rt with
+// one tile) holds a cos block and a sin block of 16 rows x PC floats and a 16 x 4 block for
+// x_0, x_{T/2}.
 // Results: ac[i][t][r] = Re X and as[i][t][r] = Im X of bin 4 g + r for pixel
 // p0w + 64 i + 4 (lane & 15) + t.  The ring of K steps (two frames per tile each) runs on into
 // the next row as in DftStream.
@@ -408,24 +409,6 @@ template <int TPW, int T, int K> struct DftStreamFold {
   }
 };
 
-// Stages the folded operand matrix of DftStreamFold from the interleaved (cos, -sin) rows
-// wm[2b][n], wm[2b+1][n] (b < 16, n < T): cos block C[b][p-1] and sin block S[b][p-1] for the
-// pairs p = 1 .. T/2-1 (zero beyond), then X[b] = (cos 0, cos(pi k), 0, 0) for x_0, x_{T/2}.
-template <int T>
-__device__ __forceinline__ void stage_fold_matrix(float *l, const float *wm, int tid, int nthreads) {
-  constexpr int PC = T / 2 + 4;
-  for (int i = tid; i < 16 * PC; i += nthreads) {
-    const int b = i / PC, j = i % PC;
-    const bool in = j < T / 2 - 1;
-    l[i] = in ? wm[(2 * b) * T + j + 1] : 0.f;
-    l[16 * PC + i] = in ? wm[(2 * b + 1) * T + j + 1] : 0.f;
-  }
-  for (int i = tid; i < 64; i += nthreads) {
-    const int b = i / 4, k = i % 4;
-    l[32 * PC + i] = k == 0 ? wm[(2 * b) * T] : k == 1 ? wm[(2 * b) * T + T / 2] : 0.f;
-  }
-}
-
 // Runtime-T form of the folded FP32 time DFT (see DftStreamFold) for NBF 16-bin tiles and one
 // 64-pixel tile per wave (stand-alone time DFT): steps of four frame pairs in batches of K,
 // T % (8 K) == 0.  Operand matrix from stage_fold_matrix_rt with pitch T/2 + 4.
@@ -486,8 +469,10 @@ __device__ __forceinline__ void dft_fold_tile(const DftRing &r, const float *lds
   }
 }
 
-// Operand matrix of dft_fold_tile from the interleaved rows wm[2b][n], wm[2b+1][n] (b < nbc;
-// rows of bins >= nbc are zero): nbf tiles of 16 bins.
+// Operand matrix of the folded DFT (DftStreamFold, dft_fold_tile) from the interleaved rows
+// wm[2b][n] = cos, wm[2b+1][n] = -sin (b < nbc): for nbf tiles of 16 bins, a cos block
+// C[b][p-1] and a sin block S[b][p-1] for the pairs p = 1 .. T/2-1 (zero beyond and for bins
+// >= nbc), pitch T/2 + 4, then X[b] = (cos 0, cos(pi k), 0, 0) for x_0, x_{T/2}.
 __device__ __forceinline__ void stage_fold_matrix_rt(float *l, const float *wm, int T, int nbf, int nbc, int tid,
                                                      int nthreads) {
   const int PC = T / 2 + 4, rows = 16 * nbf;
