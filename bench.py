@@ -5,8 +5,10 @@ Workload (the headline configuration): a 4096-frame synthetic uint16 (12-bit) st
 1024x1024 frames (disk radius 400 px, f_s = 37 kHz), resident in HBM; Fresnel propagation
 (lambda 852 nm, pitch 12 um, z 0.45 m); sliding time window T = 64, stride 16 -> 253
 windows per step; band 6-15 kHz (bins [11, 26), 15 planes); M0/M1/M2 with fftshift.
-One step = the fused path over all 253 windows.  value = moment frames (windows) per
-second over the timed steps.
+One step = the fused path over all 253 windows (one hfx_moments_batch call, which by default
+alternates the windows over two streams so that a window's front kernel overlaps the previous
+window's back kernel; --batch-lanes 1 keeps one).  value = moment frames (windows) per second
+over the timed steps.
 
 Also reported on the same line: the mixed-precision MFMA time-DFT variants (throughput and
 relative L2 of every moment image against the FP64 oracle), per-window GPU latency
