@@ -108,16 +108,24 @@ void launch_flatfield_table(float2 *out, int H, int W, float sigma_y, float sigm
 // subtracts, so each input element is read from HBM about once (halo re-reads hit L2).
 constexpr int FF_TX = 64, FF_TY = 16, FF_RMAX = 31, FF_NT = 256;
 
+// The tile buffers are sized for the actual radii (dynamic LDS, ff_spatial_lds_bytes): at
+// sigma 3 (radius 9) a block holds 20 KB instead of the 59 KB of the largest radius, so
+// eight blocks share a CU instead of two.
+__host__ __device__ constexpr size_t ff_spatial_lds_bytes(int rx, int ry) {
+  return (size_t)(FF_TY + 2 * ry) * ((FF_TX + 2 * rx) + FF_TX) * sizeof(float);
+}
+
 __global__ __launch_bounds__(FF_NT) void k_flatfield_spatial(const float *__restrict__ in, float *__restrict__ out,
                                                               int H, int W, const float *__restrict__ kx, int rx,
                                                               const float *__restrict__ ky, int ry) {
-  __shared__ float raw[(FF_TY + 2 * FF_RMAX) * (FF_TX + 2 * FF_RMAX)];
-  __shared__ float hor[(FF_TY + 2 * FF_RMAX) * FF_TX];
+  extern __shared__ float ff_lds[];
   __shared__ float wx[2 * FF_RMAX + 1], wy[2 * FF_RMAX + 1];
   const int tid = threadIdx.x;
   const int x0 = blockIdx.x * FF_TX, y0 = blockIdx.y * FF_TY;
   const long long base = (long long)blockIdx.z * H * W;
   const int rows = FF_TY + 2 * ry, pitch = FF_TX + 2 * rx;
+  float *raw = ff_lds;                 // [rows][pitch]
+  float *hor = ff_lds + rows * pitch;  // [rows][FF_TX]
   for (int i = tid; i <= 2 * rx; i += FF_NT) wx[i] = kx[i];
   for (int i = tid; i <= 2 * ry; i += FF_NT) wy[i] = ky[i];
   for (int i = tid; i < rows * pitch; i += FF_NT) {
@@ -179,7 +187,7 @@ int launch_flatfield_spatial(const float *in, float *out, float *tmp, long long 
   if (flatfield_spatial_fused(rx, ry)) {
     if (planes > 65535) return -1;
     hipLaunchKernelGGL(k_flatfield_spatial, dim3((W + FF_TX - 1) / FF_TX, (H + FF_TY - 1) / FF_TY, (unsigned)planes),
-                       dim3(FF_NT), 0, st, in, out, H, W, kx, rx, ky, ry);
+                       dim3(FF_NT), ff_spatial_lds_bytes(rx, ry), st, in, out, H, W, kx, rx, ky, ry);
   } else {
     if (!tmp) return -1;
     const long long total = planes * H * W;
