@@ -242,6 +242,37 @@ def test_two_lanes_match_one_lane(gpu, prop, W, nwin, prec):
     assert ff_default.workspace_bytes == hfx.MomentPlan(cfg).workspace_bytes
 
 
+def test_two_lane_batch_in_hip_graph(gpu):
+    """hfx_moments_batch with two lanes under HIP graph capture: the fork to the plan's stream
+    and the join back become graph edges; every replay (with eager work and synchronisation
+    between replays) must equal the eager batch bit for bit."""
+    from holoflow_amd import hfx
+
+    T, S, F = 64, 16, 160
+    p = O.PathParams(64, 512, T, S, 11, 26, "fresnel", 852e-9, 12e-6, 12e-6, 0.45, fftshift=True)
+    ring = _frames(64, 512, F)
+    plan = hfx.MomentPlan(_cfg(p))
+    assert plan.batch_lanes == 2
+    first = F - 24
+    ref = plan.moments_batch(ring, first, 5)
+    out = torch.empty_like(ref)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        plan.moments_batch(ring, first, 5, out, stream=side)  # warm-up off the default stream
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        plan.moments_batch(ring, first, 5, out)
+    for _ in range(3):
+        out.zero_()
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref)
+
+
 @pytest.mark.parametrize("prop,W", [("fresnel", 512), ("asm", 64)])
 def test_stages_compose_to_moments(gpu, prop, W):
     # stage 0 then stage 1 is hfx_moments (fused front kernel at 512, unfused at 64)
