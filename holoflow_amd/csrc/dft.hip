@@ -283,14 +283,17 @@ __global__ __launch_bounds__(row_threads<W>()) void k_dft_row(DftArgs a) {
 }
 
 // ----------------------------------------------------------------------- pipelined front
-// Same result as k_dft_row for a single LDS group of planes, with the waves split by role:
-// waves 0-3 (producers) run the MFMA time DFT of the block's next row into registers while
-// waves 4-7 (consumers) run the row FFTs of the current row's planes out of LDS and store
-// them; then the producers scatter the next row into LDS.  HBM loads and MFMAs of one row
-// overlap the LDS/VALU work and the stores of the previous one.
+// Same result as k_dft_row for a single LDS group of planes (up to FP32 rounding where the
+// DFT is folded), with the waves split by role: W/128 producer waves (8 at W = 1024) run the
+// MFMA time DFT of the block's next row into registers while four consumer waves run the row
+// FFTs of the current row's planes out of LDS and store them; then the producers scatter the
+// next row into LDS.  HBM loads and MFMAs of one row overlap the LDS/VALU work and the stores
+// of the previous one.
 // Producers own 128 pixels each (two 64-pixel MFMA tiles) and stream their frame loads
-// through a K-step register ring that runs on across rows (DftStream); four consumer waves
-// share the planes.  FP32 only, for T in {32, 64, 128}.
+// through a register ring that runs on across rows: DftStream (interleaved cos/-sin rows, up
+// to 8 bins) or DftStreamFold (9..16 bins, real-input symmetry folded).  FP32 only, for T in
+// {32, 64, 128}.  HFX_FRONT_CW / HFX_FRONT_RING override the consumer-wave count and the ring
+// depth at compile time (tuning builds; the defaults are the measured best).
 template <int W> constexpr int front_pw() { return W / 128; }
 #ifndef HFX_FRONT_CW
 #define HFX_FRONT_CW 4
