@@ -1,9 +1,11 @@
-// Time DFT of a uint16 frame window on MFMA (see dft_core.hh), in two forms:
+// Time DFT of a uint16 frame window on MFMA (see dft_core.hh), in these forms:
 //
 //   k_dft          : stand-alone; writes the band planes [planes][H][W] in natural layout
 //                    (optionally times the Fresnel lens; negative-band conjugate planes for
 //                    the two-sided FFT path).  Used for the per-stage entry point and for
-//                    widths the fused kernel does not cover.
+//                    widths the fused kernels do not cover.
+//   k_dft_fold     : the same for FP32 with 9..64 bins and T % 32 == 0, with the real-input
+//                    symmetry folded into the contraction (about half the MFMAs).
 //   k_dft_row      : fused time DFT + forward row FFT.  A persistent workgroup walks a
 //                    set of image rows; for each row its waves (64 or 128 pixels each) keep
 //                    the band spectra in MFMA registers, push them through LDS into one
@@ -11,6 +13,8 @@
 //                    the result once, column-blocked, for the column pass.  The next row's
 //                    first frames are loaded while the current row is transformed.  The
 //                    band planes never make an HBM round trip in natural layout.
+//   k_front        : the FP32 form of k_dft_row with the waves split into producers (DFT)
+//                    and consumers (row FFTs) that overlap across rows (the main path).
 //
 // Plane q of the output: q < nb -> bin kb+q; q >= nb (two-sided path) -> conj of computed
 // bin 2*nb - q, i.e. X[T - (kb + 2*nb - q)] of a real input (fft.cu:163-194 +
