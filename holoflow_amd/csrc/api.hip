@@ -595,6 +595,7 @@ hfx_status hfx_fresnel_create(int64_t batch, int32_t height, int32_t width, doub
   if (!(lambda_m > 0)) return fail(HFX_E_INVALID, "wavelength must be positive");
   if (!(dx_m > 0) || !(dy_m > 0)) return fail(HFX_E_INVALID, "pixel pitch must be positive");
   if (dx_m != dy_m) return fail(HFX_E_INVALID, "dx must equal dy");
+  if (!(z_m != 0)) return fail(HFX_E_INVALID, "Fresnel z must be non-zero");
   auto *f = new (std::nothrow) hfx_fresnel();
   if (!f) return fail(HFX_E_NOMEM, "host allocation failed");
   std::unique_ptr<hfx_fresnel> guard(f);
@@ -627,6 +628,7 @@ hfx_status hfx_fresnel_create(int64_t batch, int32_t height, int32_t width, doub
 
 hfx_status hfx_fresnel_update_z(hfx_fresnel *f, double z_m, hfx_stream_t stream) {
   if (!f) return fail(HFX_E_INVALID, "null argument");
+  if (!(z_m != 0)) return fail(HFX_E_INVALID, "Fresnel z must be non-zero");
   f->z = z_m;
   return fresnel_tables(f, S(stream));
 }

@@ -43,12 +43,19 @@ struct DftArgs {
   long long planes_stride;
 };
 
-// value of computed bin `bin` for 4 pixels, with the centring offset restored on bin 0
+// true for a computed bin that is the DC bin: bin 0, or bin T itself, the extra bin the
+// two-sided path computes when its band ends at T (its conjugate plane is X[0])
+__device__ __forceinline__ bool dc_bin(const DftArgs &a, int bin) {
+  const int k = a.kb + bin;
+  return k == 0 || k == a.r.T;
+}
+
+// value of computed bin `bin` for 4 pixels, with the centring offset restored on the DC bin
 __device__ __forceinline__ void dft_bin_values(const DftArgs &a, const f4 (&acc)[4], int h, int bin,
                                                float2 (&v)[4]) {
 #pragma unroll
   for (int t = 0; t < 4; ++t) v[t] = make_float2(acc[t][2 * h], acc[t][2 * h + 1]);
-  if (a.kb + bin == 0) {
+  if (dc_bin(a, bin)) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) v[t].x += 2048.0f * (float)a.r.T;
   }
@@ -152,7 +159,7 @@ __global__ __launch_bounds__(256) void k_dft_fold(DftArgs a) {
       float2 v[4];
 #pragma unroll
       for (int t = 0; t < 4; ++t) v[t] = make_float2(ac[f][t][rr], as[f][t][rr]);
-      if (a.kb + bin == 0) {
+      if (dc_bin(a, bin)) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) v[t].x += 2048.0f * (float)a.r.T;
       }
@@ -408,7 +415,7 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
           float2 v[4];
 #pragma unroll
           for (int t = 0; t < 4; ++t) v[t] = make_float2(fre[i][t][r], fim[i][t][r]);
-          if (a.kb + bin == 0) {  // the centring offset restored on bin 0
+          if (dc_bin(a, bin)) {  // the centring offset restored on the DC bin
 #pragma unroll
             for (int t = 0; t < 4; ++t) v[t].x += 2048.0f * (float)a.r.T;
           }

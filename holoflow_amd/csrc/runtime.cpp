@@ -571,17 +571,23 @@ bool Scheduler::run_section_once(const Section &sec, Mode mode) {
   // 2. pop the async nodes feeding this section
   for (size_t i = 0; i < sec.async_cons.size(); ++i) {
     const size_t v = sec.async_cons[i];
-    OpResult r;
-    try {
-      r = run_async_cons(v, mode);
-    } catch (const std::exception &e) {
-      throw std::runtime_error(where(v, "execute_async_consumer") + e.what());
+    OpResult r = OpResult::Ok;
+    if (!rts_[v].held) {
+      try {
+        r = run_async_cons(v, mode);
+      } catch (const std::exception &e) {
+        throw std::runtime_error(where(v, "execute_async_consumer") + e.what());
+      }
     }
-    if (r == OpResult::NotReady) {  // Once mode: give back what was popped, skip the section
-      for (size_t w : produced) release_owned_outputs(w);
+    if (r == OpResult::NotReady) {
+      // Once mode: skip the section.  Releasing the consumers already popped would consume
+      // their windows (a queue's release advances its read index); they are held instead, and
+      // the next iteration runs on the same windows.
+      for (size_t w : produced) rts_[w].held = true;
       return false;
     }
     if (r != OpResult::Ok) return false;
+    rts_[v].held = false;
     produced.push_back(v);
   }
   // 3. synchronous nodes in topological order, then the section's stream

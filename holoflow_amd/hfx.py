@@ -28,6 +28,46 @@ def _ptr(t: torch.Tensor) -> C.c_void_p:
     return C.c_void_p(t.data_ptr())
 
 
+_U16_STORAGE = (torch.int16, torch.uint16)
+
+
+def check_ring(ring: torch.Tensor, height: int, width: int, window: int) -> None:
+    """The C ABI receives only a pointer and ring.shape[0]: the tensor must be a contiguous
+    [F][H][W] 16-bit frame ring of at least `window` frames on the device, or the kernels would
+    read past it."""
+    if ring.dtype not in _U16_STORAGE:
+        raise ValueError(f"ring must hold uint16 frames (int16/uint16 storage), got {ring.dtype}")
+    if ring.dim() != 3 or tuple(ring.shape[1:]) != (height, width):
+        raise ValueError(f"ring must have shape [frames, {height}, {width}], got {tuple(ring.shape)}")
+    if ring.shape[0] < window:
+        raise ValueError(f"ring holds {ring.shape[0]} frames, fewer than the window {window}")
+    if not ring.is_contiguous():
+        raise ValueError("ring must be contiguous")
+    if not ring.is_cuda:
+        raise ValueError("ring must be a device tensor")
+
+
+def check_out(out: torch.Tensor, n_windows: int, shape: Sequence[int]) -> None:
+    """`out` must be a contiguous float32 device tensor holding n_windows (0: one window, no
+    leading axis) moment stacks of `shape` = (n_out, OH, OW)."""
+    per = 1
+    for v in shape:
+        per *= int(v)
+    if out.dtype != torch.float32:
+        raise ValueError(f"out must be float32, got {out.dtype}")
+    if not out.is_contiguous():
+        raise ValueError("out must be contiguous")
+    if n_windows == 0:
+        if out.numel() < per:
+            raise ValueError(f"out holds {out.numel()} floats, fewer than one window's {per}")
+    else:
+        if out.dim() < 1 or out.shape[0] < n_windows or out[0].numel() != per:
+            raise ValueError(f"out must have shape [>= {n_windows}, {', '.join(str(int(v)) for v in shape)}], "
+                             f"got {tuple(out.shape)}")
+    if not out.is_cuda:
+        raise ValueError("out must be a device tensor")
+
+
 @dataclass
 class MomentConfig:
     """Settings of the fused Doppler-moment plan (mirror of hfx_plan_desc)."""
@@ -116,14 +156,21 @@ class MomentPlan:
     def moments(self, ring: torch.Tensor, first_frame: int = 0, out: Optional[torch.Tensor] = None,
                 stream=None) -> torch.Tensor:
         """ring: device uint16 [F][H][W] (int16 storage is accepted bit-for-bit)."""
+        check_ring(ring, self.cfg.height, self.cfg.width, self.cfg.window)
         out = self.new_output() if out is None else out
+        check_out(out, 0, self.out_shape)
         check(lib().hfx_moments(self._h, _ptr(ring), ring.shape[0], first_frame, _ptr(out),
                                 _stream(stream)))
         return out
 
     def moments_batch(self, ring: torch.Tensor, first_frame: int, n_windows: int,
                       out: Optional[torch.Tensor] = None, stream=None) -> torch.Tensor:
+        check_ring(ring, self.cfg.height, self.cfg.width, self.cfg.window)
+        if n_windows < 0:
+            raise ValueError("n_windows must be >= 0")
         out = self.new_output(n_windows) if out is None else out
+        if n_windows:
+            check_out(out, n_windows, self.out_shape)
         stride = out[0].numel() if n_windows else 0
         check(lib().hfx_moments_batch(self._h, _ptr(ring), ring.shape[0], first_frame, n_windows,
                                       stride, _ptr(out), _stream(stream)))
@@ -132,6 +179,10 @@ class MomentPlan:
     def run_stage(self, stage: int, ring: Optional[torch.Tensor], first_frame: int = 0,
                   out: Optional[torch.Tensor] = None, stream=None) -> None:
         """Stage 0 (time DFT + row FFT into the workspace) or stage 1 (rest -> out)."""
+        if ring is not None:
+            check_ring(ring, self.cfg.height, self.cfg.width, self.cfg.window)
+        if out is not None:
+            check_out(out, 0, self.out_shape)
         rp = _ptr(ring) if ring is not None else None
         op = _ptr(out) if out is not None else None
         n = ring.shape[0] if ring is not None else 0
@@ -139,6 +190,7 @@ class MomentPlan:
 
     def time_dft(self, ring: torch.Tensor, first_frame: int = 0, stream=None) -> torch.Tensor:
         """Runs the time-DFT stage only: [n_planes][H][W] complex64 (lens applied for Fresnel)."""
+        check_ring(ring, self.cfg.height, self.cfg.width, self.cfg.window)
         out = torch.empty((self.n_planes, self.cfg.height, self.cfg.width), dtype=torch.complex64,
                           device="cuda")
         check(lib().hfx_plan_time_dft(self._h, _ptr(ring), ring.shape[0], first_frame, _ptr(out),

@@ -1,8 +1,9 @@
 // RAII and error wrappers over the HIP runtime, in the shape of the reference's curaii
 // layer (src/curaii/include/curaii/cuda.hh:29-97: CudaError + CUDA_CHECK, unique
 // device/host pointers with make_unique_*, a stream owner).  Here: HipError + HIP_CHECK,
-// and HfxError + HFX_CHECK turning a libhfx status into the exception the reference's
-// nodes throw (std::invalid_argument for invalid settings, HipError otherwise).
+// and HFX_CHECK turning a libhfx status into the exception the reference's nodes throw:
+// std::invalid_argument for invalid settings, HipError for a HIP runtime failure (the
+// reference's CudaError), HfxError for an unsupported configuration or an exhausted device.
 #pragma once
 
 #include <hip/hip_runtime_api.h>
@@ -21,6 +22,8 @@ class HipError : public std::runtime_error {
       : std::runtime_error(std::string(expr) + " failed at " + file + ":" + std::to_string(line) + ": " +
                            hipGetErrorString(e)),
         code(e) {}
+  // a HIP failure reported through libhfx's C ABI, which carries its text but not the code
+  explicit HipError(const std::string &msg) : std::runtime_error(msg), code(hipErrorUnknown) {}
   hipError_t code;
 };
 
@@ -40,6 +43,7 @@ inline void hfx_check(int status, const char *expr) {
   if (status == HFX_OK) return;
   const std::string msg = std::string(expr) + ": " + hfx_last_error();
   if (status == HFX_E_INVALID) throw std::invalid_argument(msg);
+  if (status == HFX_E_HIP) throw HipError(msg);
   throw HfxError(status, msg);
 }
 

@@ -117,6 +117,10 @@ class Scheduler {
     core::ISyncTask *sync = nullptr;
     core::IAsyncTask *async = nullptr;
     std::vector<core::TView> in_views, out_views;
+    // Once mode: an async consumer popped by a section iteration that stopped before it ran
+    // (a later consumer was not ready); its output views stay valid and unreleased, and the
+    // next iteration uses them instead of popping again
+    bool held = false;
   };
   struct Acc {
     std::atomic<uint64_t> duration_ns{0}, runs{0}, host_bytes{0}, device_bytes{0};

@@ -109,3 +109,45 @@ def test_threaded_run_metrics(gpu, tmp_path, load_kind):
     out = g.output("doppler_moments_0").cpu().numpy()
     assert rel_l2(out, O.doppler_moments(frames[:T], _params("fresnel"), "f64")) < 1e-5
     g.close()
+
+
+def test_step_holds_a_popped_window_when_a_later_queue_lags(gpu, tmp_path):
+    """Once mode (step()): a section fed by two queues pops the first, finds the second not
+    ready and is skipped; the first queue's window must stay in the queue (held, not released:
+    a release advances the read index and would drop it), so the section later runs on the
+    windows the two queues hand out in order.  Q1 hands out 1 frame per run and Q2 2 frames, so
+    run i multiplies frame i by frames 2i, 2i+1 (products of 12-bit values are exact in F32)."""
+    rng = np.random.default_rng(5)
+    frames = rng.integers(0, 4096, size=(16, 8, 16)).astype(np.uint16)
+    p = tmp_path / "lag.holo"
+    p.write_bytes(OH.pack(frames, {"version": 1}))
+    spec = {"nodes": {
+        "source": {"type": "Holofile", "params": {"path": str(p), "load_kind": "GPUCached", "start_frame": 0,
+                                                  "end_frame": 16, "batch_size": 1, "keep_cursor": False}},
+        "conv_a": {"type": "Conversion", "params": {"target": "F32", "strategy": "Real"}},
+        "conv_b": {"type": "Conversion", "params": {"target": "F32", "strategy": "Real"}},
+        "q1": {"type": "BatchQueue", "params": {"target_capacity": 4, "output_size": 1, "output_stride": 1}},
+        "q2": {"type": "BatchQueue", "params": {"target_capacity": 4, "output_size": 2, "output_stride": 2}},
+        "product": {"type": "Multiply", "params": {}},
+    }, "edges": [
+        {"from": "source", "to": "conv_a", "out": 0, "in": 0},
+        {"from": "source", "to": "conv_b", "out": 0, "in": 0},
+        {"from": "conv_a", "to": "q1", "out": 0, "in": 0},
+        {"from": "conv_b", "to": "q2", "out": 0, "in": 0},
+        {"from": "q1", "to": "product", "out": 0, "in": 0},
+        {"from": "q2", "to": "product", "out": 0, "in": 1},
+    ]}
+    g = Graph(spec)
+    runs = 0
+    for step in range(12):
+        before = g.metrics()["product"]["sample_count"]
+        g.step()
+        if g.metrics()["product"]["sample_count"] == before:
+            continue
+        out = g.output("product").cpu().numpy()
+        f = frames.astype(np.float32)
+        exp = f[runs][None] * f[2 * runs:2 * runs + 2]
+        assert np.array_equal(out, exp), f"run {runs} (step {step})"
+        runs += 1
+    assert runs >= 4
+    g.close()

@@ -315,5 +315,101 @@ def test_invalid_arguments_raise(gpu):
         hfx.MomentPlan(hfx.MomentConfig(64, 64, 24, 8, 2, 9, precision="fp16"))  # T % 32
     plan = hfx.MomentPlan(hfx.MomentConfig(64, 64, 32, 8, 6, 13))
     ring = _frames(64, 64, 16)
-    with pytest.raises(HfxInvalidArgument):
-        plan.moments(ring, 0)                                          # ring < window
+    with pytest.raises(ValueError):
+        plan.moments(ring, 0)                                          # ring < window (wrapper)
+    import ctypes as C
+
+    from holoflow_amd._lib import check, lib
+    out = plan.new_output()
+    with pytest.raises(HfxInvalidArgument):                            # the C ABI's own check
+        check(lib().hfx_moments(plan._h, C.c_void_p(ring.data_ptr()), ring.shape[0], 0,
+                                C.c_void_p(out.data_ptr()), hfx._stream(None)))
+    ring = _frames(64, 64, 40)
+    with pytest.raises(ValueError):
+        plan.moments(ring[:, :, :32].contiguous(), 0)                  # wrong frame width
+    with pytest.raises(ValueError):
+        plan.moments(ring.float(), 0)                                  # not 16-bit frames
+    with pytest.raises(ValueError):
+        plan.moments(ring, 0, torch.empty(3, 64, 32, device="cuda"))   # output too small
+    with pytest.raises(ValueError):
+        plan.moments_batch(ring, 0, 2, torch.empty(1, 3, 64, 64, device="cuda"))  # one window short
+
+
+def _focus_frames(H, W, T, k0, lam, dx, z, amp=1800.0):
+    """uint16 frames whose time-DFT bin k0 is (amp T / 2) exp(-i pi (x^2 + y^2) / (lambda z)):
+    x_n = round(2048 + amp cos(2 pi k0 n / T - phi)), phi the converging wave's phase with the
+    optical axis through the sensor centre; the other bins hold only quantisation noise."""
+    x = (np.arange(W) - W / 2.0) * dx
+    y = (np.arange(H) - H / 2.0) * dx
+    phi = np.pi * (y[:, None] ** 2 + x[None, :] ** 2) / (lam * z)
+    n = np.arange(T, dtype=np.float64)[:, None, None]
+    v = np.rint(2048.0 + amp * np.cos(2 * np.pi * k0 * n / T - phi[None]))
+    return v.astype(np.uint16)
+
+
+@pytest.mark.parametrize("H,W,T,S,kb,ke,k0,z", [(64, 64, 32, 8, 6, 13, 9, 0.45),
+                                                (1024, 1024, 64, 16, 11, 26, 18, 0.45)])
+def test_fused_fresnel_focuses_a_converging_wave(gpu, H, W, T, S, kb, ke, k0, z):
+    """The fused plan's Fresnel path (lens applied on the time-DFT output inside the front
+    kernel, fftshift on the moment store) on frames whose band bin k0 is a spherical wave
+    converging to the axis at distance z: the lens cancels it (fresnel_diffraction.cu:326-341,
+    CUFFT_FORWARD at :427; Goodman eq. 4-17), so M0 is one bright pixel at the fftshifted
+    zero frequency (H/2, W/2) worth (amp T/2) H W / nb.  A sign slip or a centre off by a pixel
+    in the lens would spread or move it."""
+    from holoflow_amd import hfx
+
+    lam, dx = 852e-9, 12e-6
+    amp = 1800.0
+    win = _focus_frames(H, W, T, k0, lam, dx, z, amp)
+    p = O.PathParams(H, W, T, S, kb, ke, "fresnel", lam, dx, dx, z, fftshift=True)
+    plan = hfx.MomentPlan(_cfg(p))
+    ring = torch.from_numpy(win.view(np.int16)).cuda()
+    out = plan.moments(ring, 0).cpu().numpy().astype(np.float64)
+    m0 = out[0]
+    nb = ke - kb
+    assert np.unravel_index(np.argmax(m0), m0.shape) == (H // 2, W // 2)
+    np.testing.assert_allclose(m0[H // 2, W // 2], amp * T / 2 * H * W / nb, rtol=1e-3)
+    assert m0.max() / m0.sum() > 0.5
+    ref = O.doppler_moments(win, p, "f64")
+    errs = [rel_l2(out[i], ref[i]) for i in range(3)]
+    assert max(errs) < TOL_FP32, errs
+
+
+@pytest.mark.parametrize("W,T,kb,ke,precision,tol", [(512, 32, 20, 32, "fp32", TOL_FP32),       # fused, folded
+                                                      (64, 32, 20, 32, "fp32", TOL_FP32),        # stand-alone, folded
+                                                      (64, 32, 27, 32, "fp32", TOL_FP32),        # stand-alone, 6 bins
+                                                      (1024, 64, 56, 64, "fp32", TOL_FP32),      # fused, T = 64
+                                                      (512, 32, 20, 32, "fp16_split", 1e-5)])   # fused 16-bit
+def test_both_sidebands_band_ending_at_window(gpu, W, T, kb, ke, precision, tol):
+    """Two-sided path with bin_end == T (validation.cc:47-52 allows it for TimeMethod::FFT):
+    the first negative-band plane is X[T - bin_end] = X[0], the DC bin, computed as the extra
+    bin T whose centring offset must be restored like bin 0's."""
+    p = O.PathParams(64, W, T, T // 4, kb, ke, "fresnel", 852e-9, 12e-6, 12e-6, 0.4, fftshift=True,
+                     both_sidebands=True)
+    F = T + T // 2
+    _check(p, _frames(64, W, F), F - T // 4, precision, tol=tol,
+           against=("f32", "f64") if precision == "fp32" else ("f64",))
+
+
+def test_cfg2_1024_mixed_precision_full_size(gpu):
+    """The precision workload at its own size: 1024^2 frames, T = 64, stride 16, bins [11, 26),
+    Fresnel z = 0.45 m with fftshift, the time DFT in FP16, BF16 and FP16 hi/lo split (FP32
+    accumulate), every moment image against the FP64 oracle on the same frames.  Bars: split
+    <= 1e-5, FP16 <= 1e-3, BF16 <= 2e-2; the mixed path's M0 <= 1e-4 (split and FP16)."""
+    from holoflow_amd import hfx
+
+    p = O.config_params("cfg1_1024_fresnel")
+    assert (p.height, p.window, p.stride, p.bin_begin, p.bin_end) == (1024, 64, 16, 11, 26)
+    ring = _frames(1024, 1024, 96, radius=400)
+    first = 32
+    win = ring.cpu().numpy().view(np.uint16)[first:first + p.window]
+    ref = O.doppler_moments(win, p, "f64")
+    for precision, tol in (("fp16_split", 1e-5), ("fp16", 1e-3), ("bf16", 2e-2)):
+        plan = hfx.MomentPlan(_cfg(p, precision))
+        out = plan.moments(ring, first).cpu().numpy()
+        plan.close()
+        errs = [rel_l2(out[i], ref[i]) for i in range(3)]
+        print("cfg2", precision, errs)
+        assert max(errs) < tol, (precision, errs)
+        if precision != "bf16":
+            assert errs[0] <= 1e-4, (precision, errs)

@@ -145,3 +145,31 @@ def test_small_ops_kats(gpu):
     xr = torch.tensor([[1.0, 2, 3, 4, 5, 6], [0, -1, 2, -3, 1, 4]])
     r = hfx.dft_axis(xr.cuda(), 1, 0, 4).cpu().numpy()
     np.testing.assert_allclose(r, np.fft.rfft(xr.numpy().astype(np.float64), axis=1), rtol=1e-4, atol=1e-5)
+
+
+def _converging_wave(H, W, lam, dx, z):
+    """exp(-i pi (x^2 + y^2) / (lambda z)), optical axis through the sensor centre."""
+    x = (np.arange(W) - W / 2.0) * dx
+    y = (np.arange(H) - H / 2.0) * dx
+    return np.exp(-1j * np.pi * (y[:, None] ** 2 + x[None, :] ** 2) / (lam * z))
+
+
+@pytest.mark.parametrize("H,W,z", [(64, 64, 0.45), (48, 96, 0.2), (1024, 1024, 0.45)])
+def test_fresnel_node_focuses_a_converging_wave(gpu, H, W, z):
+    """Fixes the sign and the centre of the lens (fresnel_diffraction.cu:326-341 with the
+    CUFFT_FORWARD transform at :427; Goodman eq. 4-17): a spherical wave converging to the
+    axis at distance z cancels the lens, so |FFT2| is one bin at (0, 0) holding H*W.  A sign
+    slip would leave the doubled chirp spread over the plane; a centre off by a pixel would
+    move the focus."""
+    from holoflow_amd import hfx
+
+    lam, dx = 852e-9, 12e-6
+    u = _converging_wave(H, W, lam, dx, z).astype(np.complex64)[None]
+    f = hfx.Fresnel(1, H, W, lam, dx, dx, z, output_magnitude=True)
+    P = f(_dev(u)).cpu().numpy()[0].astype(np.float64)
+    assert np.unravel_index(np.argmax(P), P.shape) == (0, 0)
+    np.testing.assert_allclose(P[0, 0], H * W, rtol=1e-5)
+    assert P.max() / P.sum() > 0.999
+    p = O.PathParams(H, W, 4, 4, 0, 1, "fresnel", lam, dx, dx, z)
+    ref = O.fresnel(torch.as_tensor(u.astype(np.complex128)), p, "f64").numpy()[0]
+    assert rel_l2(P, ref) < 1e-5

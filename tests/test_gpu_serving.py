@@ -10,7 +10,7 @@ import torch
 
 from holoflow_amd import hfx, serving
 
-pytestmark = pytest.mark.gpu
+
 
 SMALL = dict(sizes=[(512, 512), (768, 768), (512, 2048)], windows=[32, 64])
 
@@ -31,6 +31,20 @@ def _eager(spec: serving.StreamSpec, n_windows: int) -> np.ndarray:
     return np.stack(outs)
 
 
+# seed 0 of the full mixed load: every frame size (512^2, 768^2, 1024^2, 2048 wide x 512) and
+# every window (32, 64, 128) appears; stream 3 is 1024^2 with T = 128, stream 0 is 2048x512
+FULL_MIX_SEED = 0
+
+
+def test_full_mix_covers_every_size_and_window():  # host only: the load's composition
+    specs = serving.mixed_streams(6, seed=FULL_MIX_SEED)
+    assert {(s.height, s.width) for s in specs} == set(serving.SIZES)
+    assert {s.window for s in specs} == set(serving.WINDOWS)
+    assert (specs[3].height, specs[3].width, specs[3].window) == (1024, 1024, 128)
+    assert (specs[0].height, specs[0].width) == (512, 2048)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("policy", ["fifo", "edf"])
 def test_alone_and_in_mix_results_identical(gpu, policy):
     specs = serving.mixed_streams(6, seed=11, **SMALL)
@@ -50,6 +64,7 @@ def test_alone_and_in_mix_results_identical(gpu, policy):
     assert (mix.latency_us > 0).all() and (mix.gpu_us > 0).all()
 
 
+@pytest.mark.gpu
 def test_eager_lanes_match_graph_replay(gpu):
     specs = serving.mixed_streams(3, seed=5, **SMALL)
     wps = 10
@@ -60,6 +75,7 @@ def test_eager_lanes_match_graph_replay(gpu):
     assert np.array_equal(g.capture, e.capture)
 
 
+@pytest.mark.gpu
 def test_admission_waits_for_memory(gpu):
     specs = [serving.StreamSpec(512, 512, 32, 8, 5, 9, 0.3, 37e3, seed=i, stream_id=i) for i in range(4)]
     need = serving.stream_bytes(specs[0])
@@ -72,6 +88,7 @@ def test_admission_waits_for_memory(gpu):
     assert min(res.admit_s[2], res.admit_s[3]) >= min(res.finish_s[0], res.finish_s[1])
 
 
+@pytest.mark.gpu
 def test_server_api_backpressure_and_outputs(gpu):
     spec = serving.StreamSpec(512, 512, 32, 8, 5, 9, 0.3, 37e3, seed=3, stream_id=0)
     srv = serving.Server(serving.server_desc(max_streams=2, n_lanes=1))
@@ -97,3 +114,23 @@ def test_server_api_backpressure_and_outputs(gpu):
     srv.release(sid)
     assert srv.info()["active_streams"] == 0
     srv.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k", [3, 0])
+def test_full_mix_alone_and_in_mix_identical(gpu, k):
+    """The full mixed load (all four frame sizes, windows 32/64/128): the 1024^2 T = 128 stream
+    and a 2048x512 stream give the same moments alone, inside the mix and through eager
+    hfx_moments calls on the same frames."""
+    specs = serving.mixed_streams(6, seed=FULL_MIX_SEED)
+    wps = 8
+    arrivals = np.concatenate([[0.0], np.cumsum(np.random.default_rng(2).exponential(0.002, 5))])
+    rates = [2000.0] * len(specs)
+    mix = serving.run_workload(specs, arrivals, rates, wps, capture_stream=k,
+                               server=serving.server_desc(max_streams=6, n_lanes=4), time_limit_s=90)
+    assert mix.windows_retired == len(specs) * wps
+    assert mix.max_concurrent >= 3
+    alone = serving.run_workload([specs[k]], [0.0], [rates[k]], wps, capture_stream=0,
+                                 server=serving.server_desc(max_streams=1, n_lanes=1), time_limit_s=90)
+    assert np.array_equal(alone.capture, mix.capture)
+    assert np.array_equal(alone.capture, _eager(specs[k], wps))

@@ -103,6 +103,51 @@ def test_fresnel_far_field_and_delta():
     np.testing.assert_allclose(O.fresnel(d, p, "f64").numpy(), 1.0, rtol=1e-12)
 
 
+def _converging_wave(H, W, lam, dx, dy, z, shift=(0, 0)):
+    """exp(-i pi (x^2 + y^2) / (lambda z)) on the sensor with the optical axis through its
+    centre (x = 0 at column W/2, y = 0 at row H/2), optionally displaced by `shift` pixels."""
+    x = (np.arange(W) - W / 2.0 - shift[1]) * dx
+    y = (np.arange(H) - H / 2.0 - shift[0]) * dy
+    return np.exp(-1j * math.pi * (y[:, None] ** 2 + x[None, :] ** 2) / (lam * z))
+
+
+@pytest.mark.parametrize("H,W", [(16, 16), (12, 20), (64, 48)])
+def test_fresnel_focuses_a_converging_wave(H, W):
+    # Goodman, Introduction to Fourier Optics, eq. (4-17): the Fresnel integral is the
+    # forward Fourier transform of U(x, y) exp(+i pi (x^2 + y^2) / (lambda z)), the phase the
+    # reference puts on its input before a CUFFT_FORWARD transform
+    # (fresnel_diffraction.cu:326-341, 427).  A spherical wave converging to a point at
+    # distance z on the axis, exp(-i pi (x^2 + y^2) / (lambda z)), cancels that phase
+    # exactly: the propagated field is a single bin at zero frequency (the unshifted output's
+    # (0, 0)) holding H*W.  A conjugated lens leaves the doubled chirp (spread over many
+    # bins); an optical axis one pixel off the sensor centre leaves a linear phase, which with
+    # lambda z = S dx^2 (S = max(H, W)) moves the focus by exactly one bin along that axis.
+    S = max(H, W)
+    dx = 10e-6
+    lam = 500e-9
+    z = S * dx * dx / lam
+    p = O.PathParams(H, W, 4, 4, 0, 1, "fresnel", lam, dx, dx, z)
+    u = torch.as_tensor(_converging_wave(H, W, lam, dx, dx, z)[None])
+    P = O.fresnel(u, p, "f64").numpy()[0]
+    assert np.unravel_index(np.argmax(P), P.shape) == (0, 0)
+    np.testing.assert_allclose(P[0, 0], H * W, rtol=1e-10)
+    assert P.max() / P.sum() > 1 - 1e-9
+    # the conjugated lens (a planted sign slip): the focus is gone
+    L = O.fresnel_lens(H, W, lam, dx, dx, z)
+    Pc = np.abs(np.fft.fft2(u.numpy()[0] * np.conj(L)))
+    assert Pc.max() / Pc.sum() < 0.2
+    # a converging wave one pixel off axis (along an axis of length S) leaves the linear phase
+    # exp(+2 pi i r / S): it focuses exactly one bin further, so the centre is not free either
+    for axis, n in ((0, H), (1, W)):
+        if n != S:
+            continue
+        sh = (1, 0) if axis == 0 else (0, 1)
+        v = torch.as_tensor(_converging_wave(H, W, lam, dx, dx, z, sh)[None])
+        Q = O.fresnel(v, p, "f64").numpy()[0]
+        assert np.unravel_index(np.argmax(Q), Q.shape) == sh
+        np.testing.assert_allclose(Q[sh], H * W, rtol=1e-10)
+
+
 def test_fresnel_bruteforce_and_complex_output():
     H, W = 6, 8
     rng = np.random.default_rng(3)
