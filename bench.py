@@ -45,6 +45,17 @@ def measured_traffic(kernel):
         return None
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -70,6 +81,42 @@ def time_batch(plan, ring, n_windows, out, steps, warmup, torch):
         plan.moments_batch(ring, 0, n_windows, out)
     torch.cuda.synchronize()
     return time.perf_counter() - t0
+
+
+def hbm_probe_leg(torch, stream, nbytes=4 << 30):
+    """Best rate of the library's bandwidth probe (hfx_hbm_probe) over a small sweep of grid
+    size (workgroups per CU), 16-byte vectors in flight per lane and non-temporal access,
+    for reading, writing and copying `nbytes` (copy counts read + write bytes)."""
+    import ctypes as C
+
+    from holoflow_amd._lib import check, lib
+
+    a = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    b = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    a.fill_(7)
+    sp = C.c_void_p(stream.cuda_stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    res = {}
+    for mode, name in ((0, "read"), (1, "write"), (2, "copy")):
+        best = None
+        for bpc in (2, 4, 8):
+            for unroll in (4, 8):
+                for nt in (0, 1):
+                    run = lambda: check(lib().hfx_hbm_probe(mode, C.c_void_p(a.data_ptr()),  # noqa: E731
+                                                            C.c_void_p(b.data_ptr()), nbytes, bpc, unroll, nt, sp))
+                    run()
+                    e0.record(stream)
+                    for _ in range(4):
+                        run()
+                    e1.record(stream)
+                    torch.cuda.synchronize()
+                    gbs = nbytes * (2 if mode == 2 else 1) / (e0.elapsed_time(e1) / 4 * 1e-3) / 1e9
+                    if best is None or gbs > best["gbs"]:
+                        best = {"gbs": round(gbs, 1), "blocks_per_cu": bpc, "unroll": unroll, "nontemporal": nt}
+        res[name] = best
+    del a, b
+    torch.cuda.empty_cache()
+    return res
 
 
 def cfg4_leg(args, np, torch):
@@ -117,33 +164,48 @@ def cfg4_leg(args, np, torch):
     return res
 
 
-def serving_leg(args, np, torch, load=0.7, slots=16, policy="fifo"):
+def serving_leg(args, np, torch, slots=16, policy="fifo"):
     """Mixed-stream load: sizes {512^2, 768^2, 1024^2, 2048x512}, T in {32, 64, 128}, S = T/4,
-    z ~ U(0.2, 0.6) m, f_s ~ U(20, 75) kHz, ASM.  Each stream's windows arrive as a Poisson
-    process whose rate is set so that `slots` concurrent streams offer `load` of the GPU
-    (from each stream's own calibrated window cost); streams arrive as a Poisson process
-    too and wait for a slot and for memory."""
+    z ~ U(0.2, 0.6) m, f_s ~ U(20, 75) kHz, ASM; streams arrive as a Poisson process and wait
+    for one of `slots` slots and for memory.  Each stream's camera delivers one stride every
+    S / f_s seconds (periodic); 16 such streams at their own camera rates would need several
+    GPUs, so every camera is slowed by one common factor to offer a stated fraction of one
+    GPU (from each stream's window cost measured alone), for each load in --serving-loads.
+    Latency runs from the camera time of a window's last frame to its moments on the GPU;
+    the period it is compared with is the stream's window period at the rate it ran at."""
     from holoflow_amd import serving
 
     specs = serving.mixed_streams(args.serving_streams, seed=2024)
     costs = np.array([serving.window_cost_us(s) for s in specs])
-    rates = load / (slots * costs * 1e-6)
-    life = args.serving_windows / rates
-    rng = np.random.default_rng(7)
-    gaps = rng.exponential(float(np.mean(life)) / slots, len(specs))
-    arrivals = np.concatenate([[0.0], np.cumsum(gaps[1:])])
-    res = serving.run_workload(specs, arrivals, rates, args.serving_windows,
-                               server=serving.server_desc(max_streams=slots, n_lanes=4, policy=policy), seed=11,
-                               time_limit_s=240.0)
-    summary = serving.summarize(specs, res)
-    summary.update({
-        "offered_gpu_load_at_16_streams": load,
-        "policy": policy,
-        "window_cost_us_alone_p50": round(float(np.median(costs)), 1),
-        "note": "latency = ingest of the window's last frame -> moments complete (HIP events); "
-                "window period = stride / f_s of the stream",
-    })
-    return summary
+    cam_rates = np.array([s.fs_hz / s.stride for s in specs])          # windows/s at camera rate
+    load_cam = float(slots * np.mean(costs * 1e-6 * cam_rates))        # GPUs 16 streams would need
+    res = {"streams": len(specs), "slots": slots, "windows_per_stream": args.serving_windows,
+           "policy": policy, "arrivals": "periodic camera",
+           "window_cost_us_alone_p50": round(float(np.median(costs)), 1),
+           "offered_load_at_camera_rates": round(load_cam, 3), "loads": {}}
+    best = None
+    for load in args.serving_loads:
+        alpha = load / load_cam
+        rates = alpha * cam_rates
+        life = args.serving_windows / rates
+        rng = np.random.default_rng(7)
+        gaps = rng.exponential(float(np.mean(life)) / slots, len(specs))
+        arrivals = np.concatenate([[0.0], np.cumsum(gaps[1:])])
+        out = serving.run_workload(specs, arrivals, rates, args.serving_windows,
+                                   server=serving.server_desc(max_streams=slots, n_lanes=4, policy=policy),
+                                   seed=11, time_limit_s=240.0, arrivals="periodic")
+        summ = serving.summarize(specs, out, periods_s=1.0 / rates)
+        summ["camera_slowdown"] = round(1.0 / alpha, 3)
+        res["loads"][str(load)] = summ
+        log(f"serving load {load}: {summ}")
+        if summ["latency_over_window_period_p99"] <= 1.0 and (best is None or load > best):
+            best = load
+    res["highest_load_with_p99_within_period"] = best
+    res["note"] = ("latency = camera due time of the window's last frame -> moments complete on the GPU "
+                   "(GPU events read on the host clock, never early); period = stride / (camera rate "
+                   "at the run's slowdown); offered_load_at_camera_rates = GPU time 16 streams of this "
+                   "mix need per second at their own cameras' rates")
+    return res
 
 
 def stages_leg(args, np, torch, ring, n_windows, cfg, value):
@@ -269,6 +331,7 @@ def main():
     ap.add_argument("--cfg4-frames", type=int, default=8192)
     ap.add_argument("--serving-streams", type=int, default=24)
     ap.add_argument("--serving-windows", type=int, default=128)
+    ap.add_argument("--serving-loads", type=float, nargs="+", default=[0.3, 0.5, 0.7])
     ap.add_argument("--kernel-loop", type=int, default=200,
                     help="launches of the dominant kernel timed with HIP events")
     ap.add_argument("--batch-lanes", type=int, default=0,
@@ -338,23 +401,19 @@ def main():
     log(f"front kernel (time DFT + row FFT): {front_ms * 1e3:.1f} us, {front_gbs:.0f} GB/s; "
         f"back kernel (column FFT + moments): {back_ms * 1e3:.1f} us, {back_gbs:.0f} GB/s")
 
-    # ---- copy ceiling on this GPU: the library's copy kernel over 1 GiB, device to device
-    cp_n = 1 << 28
-    src = torch.empty(cp_n, dtype=torch.float32, device="cuda").fill_(1.0)
-    dst = torch.empty_like(src)
-    cpy = lambda: hfx.copy_contiguous(src, dst, stream=stream)  # noqa: E731
-    for _ in range(3):
-        cpy()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(20):
-        cpy()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    copy_gbs = 2 * cp_n * 4 / (e0.elapsed_time(e1) / 20 * 1e-3) / 1e9
-    del src, dst
-    log(f"copy kernel: {copy_gbs:.0f} GB/s (read + write)")
+    # ---- practical HBM ceiling on this GPU: read-only / write-only / copy probes over 4 GiB
+    probe = hbm_probe_leg(torch, stream)
+    copy_gbs = probe["copy"]["gbs"]
+    log(f"HBM probe: {probe}")
 
+    # ---- MFMA utilisation of the time DFT (4 N T n_p useful flops per window, real input times
+    # complex twiddles) over the front kernel's time, against the FP32 MFMA peak
+    dft_flops = 4 * npix * T * nb
+    dft_tflops = dft_flops / (front_ms * 1e-3) / 1e12
+    dft_mfma = {"useful_flops_per_window": dft_flops, "tflops_over_front_kernel": round(dft_tflops, 2),
+                "peak_tflops_fp32_mfma": 157.3, "frac": round(dft_tflops / 157.3, 4),
+                "note": "the DFT shares the front kernel with the row FFTs (VALU); the folded form "
+                        "issues about half the MFMA flops of the direct contraction"}
     path_bytes = T * npix * 2 + 2 * nb * npix * 8 + 3 * npix * 4
     path_gbs = path_bytes * value / 1e9
     traffic = measured_traffic("k_front")
@@ -432,11 +491,19 @@ def main():
             O.doppler_moments(host[w * S:w * S + T], p_or, "f32")
             done += 1
         cdt = time.perf_counter() - t0
+        # the GPU FP32 path against the CPU FP32 restatement on the same windows
+        errs = np.zeros(3)
+        for w in range(min(4, n_pos)):
+            c = O.doppler_moments(host[w * S:w * S + T], p_or, "f32")
+            g = plan.moments(ring, w * S).cpu().numpy()
+            errs = np.maximum(errs, [O.rel_l2(g[i], c[i]) for i in range(3)])
         cpu = {"value": round(done / cdt, 4), "unit": "moment frames/s", "cores": threads,
-               "kind": "port",
+               "kind": "port", "host_cpu_count": os.cpu_count(), "cpu_model": cpu_model(),
+               "gpu_vs_cpu_fp32_rel_l2_max": {k: float(f"{errs[i]:.3e}") for i, k in enumerate(("M0", "M1", "M2"))},
                "sample": f"{done} windows (T=64, stride 16) cycling over the first {n_pos} window positions "
                          f"of the same 1024x1024 stream; FP32 torch-CPU (MKL FFT) restatement of the "
-                         f"reference graph, {threads} threads, {cdt:.1f} s"}
+                         f"reference graph, {threads} threads (the host's CPU share for one GPU), {cdt:.1f} s; "
+                         f"rel-L2 of the GPU FP32 path against it over windows 0-3"}
         log(f"cpu baseline: {cpu}")
 
     line = {
@@ -463,8 +530,10 @@ def main():
         "kernels": {"front_us": round(front_ms * 1e3, 2), "front_gbs": round(front_gbs, 1),
                     "back_us": round(back_ms * 1e3, 2), "back_gbs": round(back_gbs, 1),
                     "back_kernel": "k_wcol<1024,1> (column FFT + |.| + moments)",
-                    "back_algorithmic_bytes_per_launch": back_bytes,
-                    "copy_ceiling_gbs": round(copy_gbs, 1)},
+                    "back_algorithmic_bytes_per_launch": back_bytes},
+        "hbm_probe": dict(probe, note="hfx_hbm_probe over 4 GiB, best of a grid/unroll/non-temporal "
+                                      "sweep; copy counts read + write bytes"),
+        "dft_mfma": dft_mfma,
         "path_roofline": {"achieved": round(path_gbs, 1), "frac": round(path_gbs / HBM_PEAK_GBS, 4),
                           "algorithmic_bytes_per_window": path_bytes},
         "latency_ms": {"window_gpu_p50": round(float(np.percentile(lat, 50)), 4),

@@ -150,6 +150,7 @@ SIGNATURES = {
     "hfx_convert_u16": (C.c_int, [_VP, _VP, C.c_int64, C.c_int32, _VP]),
     "hfx_convert_scratch_bytes": (C.c_int64, [C.c_int64]),
     "hfx_convert": (C.c_int, [_VP, C.c_int32, _VP, C.c_int32, C.c_int32, C.c_int64, _VP, _VP]),
+    "hfx_hbm_probe": (C.c_int, [C.c_int32, _VP, _VP, C.c_int64, C.c_int32, C.c_int32, C.c_int32, _VP]),
     "hfx_slide_avg_step": (C.c_int, [_VP, _VP, _VP, C.c_int64, C.c_int32, _VP]),
     "hfx_synth_desc_init": (None, [_P(SynthDesc), C.c_int32, C.c_int32]),
     "hfx_synth_frames": (C.c_int, [_P(SynthDesc), _VP, C.c_int64, C.c_int64, _VP, _VP]),
@@ -183,26 +184,32 @@ class StreamDesc(C.Structure):
 
 class WindowDone(C.Structure):
     _fields_ = [("stream_id", C.c_int32), ("n_outputs", C.c_int32), ("window", C.c_int64),
-                ("latency_us", C.c_float), ("gpu_us", C.c_float)]
+                ("latency_us", C.c_float), ("gpu_us", C.c_float), ("wait_us", C.c_float)]
 
 
 class ServerInfo(C.Structure):
     _fields_ = [("active_streams", C.c_int32), ("max_streams", C.c_int32), ("bytes_held", C.c_int64),
-                ("bytes_free", C.c_int64), ("windows_launched", C.c_int64), ("windows_retired", C.c_int64)]
+                ("bytes_free", C.c_int64), ("windows_launched", C.c_int64), ("windows_retired", C.c_int64),
+                ("clock_error_us", C.c_double)]
 
 
 class WorkloadDesc(C.Structure):
     _fields_ = [("server", ServerDesc), ("n_streams", C.c_int32), ("streams", C.POINTER(StreamDesc)),
                 ("arrival_s", C.POINTER(C.c_double)), ("window_rate_hz", C.POINTER(C.c_double)),
                 ("windows_per_stream", C.c_int32), ("seed", C.c_uint64), ("capture_stream", C.c_int32),
-                ("capture_out", C.POINTER(C.c_float)), ("time_limit_s", C.c_double)]
+                ("capture_out", C.POINTER(C.c_float)), ("time_limit_s", C.c_double), ("arrivals", C.c_int32)]
+
+
+ARRIVALS_PERIODIC, ARRIVALS_POISSON = 0, 1
 
 
 class WorkloadResult(C.Structure):
     _fields_ = [("wall_s", C.c_double), ("windows_retired", C.c_int64), ("max_concurrent", C.c_int32),
                 ("admissions_deferred", C.c_int32), ("ingest_deferred", C.c_int64),
                 ("latency_us", C.POINTER(C.c_float)), ("gpu_us", C.POINTER(C.c_float)),
-                ("admit_s", C.POINTER(C.c_double)), ("finish_s", C.POINTER(C.c_double))]
+                ("admit_s", C.POINTER(C.c_double)), ("finish_s", C.POINTER(C.c_double)),
+                ("wait_us", C.POINTER(C.c_float)), ("windows_ingested_late", C.c_int64),
+                ("clock_error_us", C.c_double)]
 
 
 SIGNATURES.update({
@@ -213,7 +220,9 @@ SIGNATURES.update({
     "hfx_server_create": (C.c_int, [C.POINTER(ServerDesc), C.POINTER(C.c_void_p)]),
     "hfx_server_destroy": (C.c_int, [C.c_void_p]),
     "hfx_server_admit": (C.c_int, [C.c_void_p, C.POINTER(StreamDesc), C.POINTER(C.c_int32)]),
+    "hfx_server_clock": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "hfx_server_ingest": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64]),
+    "hfx_server_ingest_due": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_double]),
     "hfx_server_schedule": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "hfx_server_poll": (C.c_int, [C.c_void_p, C.POINTER(WindowDone), C.c_int32, C.POINTER(C.c_int32)]),
     "hfx_server_window_output": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.POINTER(C.c_void_p)]),

@@ -1010,6 +1010,19 @@ hfx_status hfx_convert(const void *in, int32_t in_dtype, void *out, int32_t out_
   return HFX_OK;
 }
 
+hfx_status hfx_hbm_probe(int32_t mode, const void *in, void *out, int64_t bytes, int32_t blocks_per_cu,
+                         int32_t unroll, int32_t nontemporal, hfx_stream_t stream) {
+  if (mode < 0 || mode > 2) return fail(HFX_E_INVALID, "mode must be 0 (read), 1 (write) or 2 (copy)");
+  if (bytes < 0 || blocks_per_cu < 1 || blocks_per_cu > 64) return fail(HFX_E_INVALID, "invalid size or grid");
+  if (!out || (mode != 1 && !in)) return fail(HFX_E_INVALID, "null buffer");
+  if ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) % 16) return fail(HFX_E_INVALID, "buffers must be 16-byte aligned");
+  const int rc = hfx::launch_hbm_probe(mode, in, out, bytes, blocks_per_cu, unroll, nontemporal,
+                                       reinterpret_cast<unsigned *>(out), S(stream));
+  if (rc == -2) return fail(HFX_E_INVALID, "unroll must be 1, 2, 4 or 8");
+  if (rc != 0) return fail(HFX_E_HIP, "probe launch failed");
+  return HFX_OK;
+}
+
 hfx_status hfx_slide_avg_step(const float *frame, float *old_slot, float *running, int64_t n, int32_t window,
                               hfx_stream_t stream) {
   if ((!frame || !old_slot || !running) && n > 0) return fail(HFX_E_INVALID, "null argument");

@@ -120,6 +120,8 @@ int launch_convert_u16(const uint16_t *in, void *out, long long n, int to_comple
 int minmax_blocks(long long n);
 int launch_convert(const void *in, int in_dt, void *out, int out_dt, int strategy, long long n,
                    void *scratch, hipStream_t st);
+int launch_hbm_probe(int mode, const void *in, void *out, long long bytes, int blocks_per_cu, int unroll,
+                     int nontemporal, unsigned *sink, hipStream_t st);
 int launch_slide_avg(const float *x, float *old_slot, float *ravg, long long n, int ws, hipStream_t st);
 int launch_dft_axis(const void *in, int dtype, long long outer, int n, long long inner, int kb,
                     int ke, float scale, float2 *out, hipStream_t st);

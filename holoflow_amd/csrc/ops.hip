@@ -292,6 +292,93 @@ __global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ in, ui
   for (; i < n16; i += stride) out[i] = in[i];
 }
 
+// HBM bandwidth probe (measurement only): mode 0 reads the buffer (each lane folds its
+// 16-byte vectors into one word, stored only if it equals a value no data here produces, so
+// the loads stay live), mode 1 writes it, mode 2 copies it.  U independent 16-byte vectors per
+// lane per iteration, grid-stride over the buffer, non-temporal for NT = 1.
+template <int MODE, int U, int NT>
+__global__ __launch_bounds__(256) void k_hbm_probe(const uint4 *__restrict__ in, uint4 *__restrict__ out,
+                                                   long long n16, unsigned *sink) {
+  const long long stride = (long long)gridDim.x * 256;
+  long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  typedef unsigned u4v __attribute__((ext_vector_type(4)));
+  unsigned acc = 0;
+  const uint4 fill = make_uint4(threadIdx.x, blockIdx.x, 0x5a5a5a5au, 0xa5a5a5a5u);
+  for (; i + (U - 1) * stride < n16; i += U * stride) {
+    u4v v[U];
+    if (MODE != 1) {
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        v[u] = NT ? __builtin_nontemporal_load(reinterpret_cast<const u4v *>(in) + i + u * stride)
+                  : reinterpret_cast<const u4v *>(in)[i + u * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (MODE == 0) {
+        acc ^= v[u].x ^ v[u].y ^ v[u].z ^ v[u].w;
+      } else {
+        const u4v w = MODE == 1 ? u4v{fill.x, fill.y, fill.z, fill.w} : v[u];
+        if (NT)
+          __builtin_nontemporal_store(w, reinterpret_cast<u4v *>(out) + i + u * stride);
+        else
+          reinterpret_cast<u4v *>(out)[i + u * stride] = w;
+      }
+    }
+  }
+  for (; i < n16; i += stride) {
+    if (MODE == 0) {
+      const uint4 v = in[i];
+      acc ^= v.x ^ v.y ^ v.z ^ v.w;
+    } else {
+      out[i] = MODE == 1 ? fill : in[i];
+    }
+  }
+  if (MODE == 0 && acc == 0x9e3779b9u) sink[blockIdx.x & 255] = acc;
+}
+
+template <int MODE, int U>
+static void launch_probe_u(int nt, unsigned blocks, const uint4 *in, uint4 *out, long long n16, unsigned *sink,
+                           hipStream_t st) {
+  if (nt)
+    hipLaunchKernelGGL((k_hbm_probe<MODE, U, 1>), dim3(blocks), dim3(256), 0, st, in, out, n16, sink);
+  else
+    hipLaunchKernelGGL((k_hbm_probe<MODE, U, 0>), dim3(blocks), dim3(256), 0, st, in, out, n16, sink);
+}
+
+template <int MODE>
+static int launch_probe_mode(int unroll, int nt, unsigned blocks, const uint4 *in, uint4 *out, long long n16,
+                             unsigned *sink, hipStream_t st) {
+  switch (unroll) {
+    case 1: launch_probe_u<MODE, 1>(nt, blocks, in, out, n16, sink, st); break;
+    case 2: launch_probe_u<MODE, 2>(nt, blocks, in, out, n16, sink, st); break;
+    case 4: launch_probe_u<MODE, 4>(nt, blocks, in, out, n16, sink, st); break;
+    case 8: launch_probe_u<MODE, 8>(nt, blocks, in, out, n16, sink, st); break;
+    default: return -2;
+  }
+  return 0;
+}
+
+int launch_hbm_probe(int mode, const void *in, void *out, long long bytes, int blocks_per_cu, int unroll,
+                     int nontemporal, unsigned *sink, hipStream_t st) {
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const long long n16 = bytes / 16;
+  if (n16 <= 0) return 0;
+  long long blocks = (long long)blocks_per_cu * cus;
+  if (blocks > (n16 + 255) / 256) blocks = (n16 + 255) / 256;
+  const uint4 *i4 = reinterpret_cast<const uint4 *>(in);
+  uint4 *o4 = reinterpret_cast<uint4 *>(out);
+  int rc = -2;
+  switch (mode) {
+    case 0: rc = launch_probe_mode<0>(unroll, nontemporal, (unsigned)blocks, i4, o4, n16, sink, st); break;
+    case 1: rc = launch_probe_mode<1>(unroll, nontemporal, (unsigned)blocks, i4, o4, n16, sink, st); break;
+    case 2: rc = launch_probe_mode<2>(unroll, nontemporal, (unsigned)blocks, i4, o4, n16, sink, st); break;
+  }
+  if (rc != 0) return rc;
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 static int launch_copy_bytes(const uint8_t *in, uint8_t *out, long long bytes, hipStream_t st) {
   int dev = 0, cus = 256;
   (void)hipGetDevice(&dev);

@@ -97,7 +97,7 @@ struct Stream {
   double *phase = nullptr;
   std::vector<hipGraphExec_t> graphs;
   std::vector<hipEvent_t> ev_ready, ev_launch, ev_done;
-  std::vector<double> ready_t;  // host time (s) at which the window of each position became ready
+  std::vector<double> due_t;  // server-clock time (s) the camera delivered each position's last frame
   int64_t frames_in = 0, launched = 0, retired = 0;
 
   int64_t ready_windows() const { return frames_in >= g.T ? (frames_in - g.T) / g.S + 1 : 0; }
@@ -134,7 +134,18 @@ struct hfx_server {
   std::vector<std::deque<std::pair<int32_t, int64_t>>> lane_windows;  // EDF: (stream, window) per lane
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   int64_t bytes_held = 0, windows_launched = 0, windows_retired = 0;
+  // GPU timestamps on the server clock: `epoch` was recorded on an idle stream and had
+  // completed by host time epoch_s, so event e completed at or before
+  // epoch_s + elapsed(epoch, e), late by at most epoch_err_s (the recording's round trip)
+  hipEvent_t epoch = nullptr;
+  double epoch_s = 0.0, epoch_err_s = 0.0;
   double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+  hipError_t gpu_time(hipEvent_t e, double &t) const {
+    float ms = 0.f;
+    const hipError_t r = hipEventElapsedTime(&ms, epoch, e);
+    t = epoch_s + 1e-3 * (double)ms;
+    return r;
+  }
 };
 
 namespace {
@@ -172,7 +183,7 @@ hfx_status build_stream(hfx_server *s, Stream &st) {
   SRV_HIP(hipMemsetAsync(st.phase, 0, (size_t)g.npix * 8, cs));
   SRV_TRY(hfx_synth_frames(&st.d.camera, st.pool, 0, g.P, st.phase, cs));
   SRV_HIP(hipStreamSynchronize(cs));
-  st.ready_t.assign(g.n_pos, 0.0);
+  st.due_t.assign(g.n_pos, 0.0);
   st.ev_ready.assign(g.n_pos, nullptr);
   st.ev_launch.assign(g.n_pos, nullptr);
   st.ev_done.assign(g.n_pos, nullptr);
@@ -264,6 +275,28 @@ hfx_status hfx_server_create(const hfx_server_desc *d, hfx_server **out) {
   SRV_HIP(hipStreamCreateWithFlags(&s->capture, hipStreamNonBlocking));
   s->lanes.assign(d->n_lanes, nullptr);
   for (auto &l : s->lanes) SRV_HIP(hipStreamCreateWithFlags(&l, hipStreamNonBlocking));
+  // epoch: of a few recordings on the idle ingest stream keep the one with the shortest
+  // host round trip; its completion time lies inside [before, after] of that round trip, and
+  // taking `after` makes every GPU time read through it late (never early) by at most the gap
+  hipEvent_t cand[8] = {};
+  double best_gap = 1e30;
+  int best = -1;
+  for (int i = 0; i < 8; ++i) {
+    SRV_HIP(hipEventCreate(&cand[i]));
+    const double before = s->now();
+    SRV_HIP(hipEventRecord(cand[i], s->ingest));
+    SRV_HIP(hipEventSynchronize(cand[i]));
+    const double after = s->now();
+    if (after - before < best_gap) {
+      best_gap = after - before;
+      best = i;
+      s->epoch_s = after;
+    }
+  }
+  for (int i = 0; i < 8; ++i)
+    if (i != best) (void)hipEventDestroy(cand[i]);
+  s->epoch = cand[best];
+  s->epoch_err_s = best_gap;
   s->lane_load.assign(d->n_lanes, 0);
   s->lane_windows.resize(d->n_lanes);
   s->slots.resize(d->max_streams);
@@ -280,6 +313,7 @@ hfx_status hfx_server_destroy(hfx_server *s) {
     if (l) (void)hipStreamDestroy(l);
   if (s->ingest) (void)hipStreamDestroy(s->ingest);
   if (s->capture) (void)hipStreamDestroy(s->capture);
+  if (s->epoch) (void)hipEventDestroy(s->epoch);
   delete s;
   return HFX_OK;
 }
@@ -322,7 +356,18 @@ hfx_status hfx_server_admit(hfx_server *s, const hfx_stream_desc *d, int32_t *st
   return HFX_OK;
 }
 
+hfx_status hfx_server_clock(hfx_server *s, double *t_s) {
+  if (!s || !t_s) return fail(HFX_E_INVALID, "null argument");
+  *t_s = s->now();
+  return HFX_OK;
+}
+
 hfx_status hfx_server_ingest(hfx_server *s, int32_t stream_id, int64_t n_frames) {
+  if (!s) return fail(HFX_E_INVALID, "null server");
+  return hfx_server_ingest_due(s, stream_id, n_frames, s->now());
+}
+
+hfx_status hfx_server_ingest_due(hfx_server *s, int32_t stream_id, int64_t n_frames, double due_s) {
   Stream *st = stream_of(s, stream_id);
   if (!st) return fail(HFX_E_INVALID, "unknown stream id");
   const Geometry &g = st->g;
@@ -336,10 +381,9 @@ hfx_status hfx_server_ingest(hfx_server *s, int32_t stream_id, int64_t n_frames)
   const int64_t old_ready = st->ready_windows();
   SRV_TRY(copy_frames(s, *st, st->frames_in, n_frames));
   st->frames_in = F;
-  const double t_ready = s->now();
   for (int64_t w = old_ready; w < st->ready_windows(); ++w) {
     SRV_HIP(hipEventRecord(st->ev_ready[w % g.n_pos], s->ingest));
-    st->ready_t[w % g.n_pos] = t_ready;
+    st->due_t[w % g.n_pos] = due_s;
   }
   return HFX_OK;
 }
@@ -373,7 +417,7 @@ static hfx_status schedule_edf(hfx_server *s, int32_t &n) {
       if (st.launched >= st.ready_windows()) continue;
       if (st.launched > 0 && window_on_gpu(st, st.launched - 1)) continue;  // one window in flight
       const double period = (double)st.g.S / (st.d.plan.fs_hz > 0 ? st.d.plan.fs_hz : 1.0);
-      const double deadline = st.ready_t[st.launched % st.g.n_pos] + period;
+      const double deadline = st.due_t[st.launched % st.g.n_pos] + period;
       if (!best || deadline < best_deadline) {
         best = &st;
         best_deadline = deadline;
@@ -393,13 +437,19 @@ hfx_status hfx_server_schedule(hfx_server *s, int32_t *launched) {
   if (s->d.policy == HFX_SCHED_EDF) {
     SRV_TRY(schedule_edf(s, n));
   } else {
+    // every ready window, oldest camera due time first across the streams (a stream's own
+    // windows keep their order: their due times do not decrease)
+    std::vector<std::pair<double, Stream *>> ready;
     for (auto &up : s->slots) {
       if (!up) continue;
       Stream &st = *up;
-      while (st.launched < st.ready_windows()) {
-        SRV_TRY(launch_window(s, st, st.lane));
-        ++n;
-      }
+      for (int64_t w = st.launched; w < st.ready_windows(); ++w) ready.emplace_back(st.due_t[w % st.g.n_pos], &st);
+    }
+    std::stable_sort(ready.begin(), ready.end(),
+                     [](const auto &a, const auto &b) { return a.first < b.first; });
+    for (auto &r : ready) {
+      SRV_TRY(launch_window(s, *r.second, r.second->lane));
+      ++n;
     }
   }
   if (launched) *launched = n;
@@ -417,14 +467,17 @@ hfx_status hfx_server_poll(hfx_server *s, hfx_window_done *out, int32_t cap, int
       const hipError_t q = hipEventQuery(st.ev_done[p]);
       if (q == hipErrorNotReady) break;
       SRV_HIP(q);
-      float lat = 0.f, gpu = 0.f;
-      SRV_HIP(hipEventElapsedTime(&lat, st.ev_ready[p], st.ev_done[p]));
+      float gpu = 0.f;
+      double t_start = 0.0, t_done = 0.0;
       SRV_HIP(hipEventElapsedTime(&gpu, st.ev_launch[p], st.ev_done[p]));
+      SRV_HIP(s->gpu_time(st.ev_launch[p], t_start));
+      SRV_HIP(s->gpu_time(st.ev_done[p], t_done));
       out[k].stream_id = st.id;
       out[k].n_outputs = (int32_t)st.g.n_out;
       out[k].window = st.retired;
-      out[k].latency_us = lat * 1e3f;
+      out[k].latency_us = (float)((t_done - st.due_t[p]) * 1e6);
       out[k].gpu_us = gpu * 1e3f;
+      out[k].wait_us = (float)((t_start - st.due_t[p]) * 1e6);
       ++k;
       ++st.retired;
       ++s->windows_retired;
@@ -468,6 +521,7 @@ hfx_status hfx_server_info_get(hfx_server *s, hfx_server_info *info) {
   info->bytes_free = (int64_t)free_b;
   info->windows_launched = s->windows_launched;
   info->windows_retired = s->windows_retired;
+  info->clock_error_us = s->epoch_err_s * 1e6;
   return HFX_OK;
 }
 
@@ -480,15 +534,23 @@ hfx_status hfx_serve_workload(const hfx_workload_desc *d, hfx_workload_result *r
     return fail(HFX_E_INVALID, "invalid capture stream");
   for (int i = 0; i < d->n_streams; ++i)
     if (!(d->window_rate_hz[i] > 0)) return fail(HFX_E_INVALID, "window rates must be positive");
+  if (d->arrivals != HFX_ARRIVALS_PERIODIC && d->arrivals != HFX_ARRIVALS_POISSON)
+    return fail(HFX_E_INVALID, "unknown window arrival process");
   hfx_server *srv = nullptr;
   SRV_TRY(hfx_server_create(&d->server, &srv));
   std::unique_ptr<hfx_server, hfx_status (*)(hfx_server *)> guard(srv, hfx_server_destroy);
 
+  // Camera model: after admission a stream's frames accumulate; window w is complete at its
+  // due time, the camera time of its last frame.  Periodic: the camera delivers one stride of
+  // frames every 1/rate (window 0 needs T/S strides); Poisson: exponential gaps of mean 1/rate
+  // (window 0 included).  A window is ingested at or after its due time (later when the ring
+  // is still full), and its latency runs from the due time, so waiting behind a full ring
+  // counts.
   struct Live {
     int32_t id = -1;
     bool admitted = false, released = false;
     int64_t arrived = 0, retired = 0;
-    double next_s = 0;
+    double due_s = 0;  // due time of window `arrived` (server clock)
   };
   const int n = d->n_streams, wps = d->windows_per_stream;
   std::vector<Live> live(n);
@@ -499,16 +561,23 @@ hfx_status hfx_serve_workload(const hfx_workload_desc *d, hfx_workload_result *r
   r->max_concurrent = 0;
   r->admissions_deferred = 0;
   r->ingest_deferred = 0;
+  r->windows_ingested_late = 0;
   int active = 0, finished = 0, next_admit = 0;
-  const auto t0 = std::chrono::steady_clock::now();
-  auto now_s = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
-  double last_retire = 0.0;
+  const double t0 = srv->now();  // arrival_s is relative to the loop's start
+  auto now_s = [&] { return srv->now(); };
+  double last_retire = t0;
+  auto next_gap = [&](int i) {
+    const double rate = d->window_rate_hz[i];
+    if (d->arrivals == HFX_ARRIVALS_PERIODIC) return 1.0 / rate;
+    std::exponential_distribution<double> ex(rate);
+    return ex(rng);
+  };
 
   while (finished < n) {
     const double now = now_s();
-    if (now > d->time_limit_s) return fail(HFX_E_HIP, "workload exceeded its time limit");
+    if (now - t0 > d->time_limit_s) return fail(HFX_E_HIP, "workload exceeded its time limit");
     // admissions, in arrival order
-    while (next_admit < n && d->arrival_s[next_admit] <= now) {
+    while (next_admit < n && t0 + d->arrival_s[next_admit] <= now) {
       Live &L = live[next_admit];
       const hfx_status rc = hfx_server_admit(srv, &d->streams[next_admit], &L.id);
       if (rc == HFX_NOT_READY) {
@@ -517,28 +586,32 @@ hfx_status hfx_serve_workload(const hfx_workload_desc *d, hfx_workload_result *r
       }
       SRV_TRY(rc);
       L.admitted = true;
-      L.next_s = now_s();
+      const double t_adm = now_s();
+      const hfx_plan_desc &p = d->streams[next_admit].plan;
+      L.due_s = t_adm + (d->arrivals == HFX_ARRIVALS_PERIODIC
+                             ? (double)(p.window / p.stride) / d->window_rate_hz[next_admit]
+                             : next_gap(next_admit));
       by_id[L.id] = next_admit;
-      if (r->admit_s) r->admit_s[next_admit] = L.next_s;
+      if (r->admit_s) r->admit_s[next_admit] = t_adm - t0;
       ++active;
       r->max_concurrent = std::max(r->max_concurrent, active);
       ++next_admit;
     }
-    // window arrivals: the camera delivers the window's new frames
+    // window arrivals: the camera has delivered the window's frames by its due time
     for (int i = 0; i < next_admit; ++i) {
       Live &L = live[i];
       if (!L.admitted || L.released) continue;
-      while (L.arrived < wps && L.next_s <= now_s()) {
+      while (L.arrived < wps && L.due_s <= now_s()) {
         const hfx_plan_desc &p = d->streams[i].plan;
-        const hfx_status rc = hfx_server_ingest(srv, L.id, L.arrived == 0 ? p.window : p.stride);
+        const hfx_status rc = hfx_server_ingest_due(srv, L.id, L.arrived == 0 ? p.window : p.stride, L.due_s);
         if (rc == HFX_NOT_READY) {
           ++r->ingest_deferred;
           break;
         }
         SRV_TRY(rc);
+        if (now_s() - L.due_s > 1.0 / d->window_rate_hz[i]) ++r->windows_ingested_late;
         ++L.arrived;
-        std::exponential_distribution<double> ex(d->window_rate_hz[i]);
-        L.next_s += ex(rng);
+        L.due_s += next_gap(i);
       }
     }
     int32_t launched = 0;
@@ -549,8 +622,10 @@ hfx_status hfx_serve_workload(const hfx_workload_desc *d, hfx_workload_result *r
       const hfx_window_done &w = done[j];
       const int i = by_id[w.stream_id];
       Live &L = live[i];
-      if (r->latency_us) r->latency_us[(int64_t)i * wps + w.window] = w.latency_us;
-      if (r->gpu_us) r->gpu_us[(int64_t)i * wps + w.window] = w.gpu_us;
+      const int64_t at = (int64_t)i * wps + w.window;
+      if (r->latency_us) r->latency_us[at] = w.latency_us;
+      if (r->gpu_us) r->gpu_us[at] = w.gpu_us;
+      if (r->wait_us) r->wait_us[at] = w.wait_us;
       if (i == d->capture_stream) {
         const float *src = nullptr;
         SRV_TRY(hfx_server_window_output(srv, w.stream_id, w.window, &src));
@@ -565,13 +640,14 @@ hfx_status hfx_serve_workload(const hfx_workload_desc *d, hfx_workload_result *r
         SRV_TRY(hfx_server_release(srv, L.id));
         by_id[L.id] = -1;
         L.released = true;
-        if (r->finish_s) r->finish_s[i] = now_s();
+        if (r->finish_s) r->finish_s[i] = now_s() - t0;
         --active;
         ++finished;
       }
     }
   }
-  r->wall_s = last_retire - d->arrival_s[0];
+  r->wall_s = last_retire - (t0 + d->arrival_s[0]);
+  r->clock_error_us = srv->epoch_err_s * 1e6;
   return HFX_OK;
 }
 

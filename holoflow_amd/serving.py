@@ -136,22 +136,29 @@ class WorkloadResult:
     max_concurrent: int
     admissions_deferred: int
     ingest_deferred: int
-    latency_us: np.ndarray      # [n_streams, windows_per_stream]
-    gpu_us: np.ndarray
+    latency_us: np.ndarray      # [n_streams, windows_per_stream]: due time -> moments complete
+    gpu_us: np.ndarray          # launch -> complete on its lane
+    wait_us: np.ndarray         # due time -> start of its work
     admit_s: np.ndarray
     finish_s: np.ndarray
     capture: Optional[np.ndarray]
+    windows_ingested_late: int = 0
+    clock_error_us: float = 0.0
 
 
 def run_workload(specs: Sequence[StreamSpec], arrival_s: Sequence[float], window_rate_hz: Sequence[float],
                  windows_per_stream: int, server=None, seed: int = 0, capture_stream: int = -1,
-                 time_limit_s: float = 120.0) -> WorkloadResult:
+                 time_limit_s: float = 120.0, arrivals: str = "poisson") -> WorkloadResult:
+    """Runs the native workload driver: stream i arrives at arrival_s[i] and its camera delivers
+    windows at window_rate_hz[i], periodically ("periodic": one stride every 1/rate) or as a
+    Poisson process ("poisson")."""
     n = len(specs)
     descs = (_lib.StreamDesc * n)(*[s.to_desc() for s in specs])
     arr = (C.c_double * n)(*[float(a) for a in arrival_s])
     rates = (C.c_double * n)(*[float(r) for r in window_rate_hz])
     lat = np.zeros((n, windows_per_stream), np.float32)
     gpu = np.zeros((n, windows_per_stream), np.float32)
+    wait = np.zeros((n, windows_per_stream), np.float32)
     adm = np.zeros(n, np.float64)
     fin = np.zeros(n, np.float64)
     cap = None
@@ -165,6 +172,7 @@ def run_workload(specs: Sequence[StreamSpec], arrival_s: Sequence[float], window
     d.seed = seed
     d.capture_stream = capture_stream
     d.time_limit_s = time_limit_s
+    d.arrivals = {"periodic": _lib.ARRIVALS_PERIODIC, "poisson": _lib.ARRIVALS_POISSON}[arrivals]
     if capture_stream >= 0:
         s = specs[capture_stream]
         n_out = 3
@@ -175,14 +183,20 @@ def run_workload(specs: Sequence[StreamSpec], arrival_s: Sequence[float], window
     r.gpu_us = gpu.ctypes.data_as(C.POINTER(C.c_float))
     r.admit_s = adm.ctypes.data_as(C.POINTER(C.c_double))
     r.finish_s = fin.ctypes.data_as(C.POINTER(C.c_double))
+    r.wait_us = wait.ctypes.data_as(C.POINTER(C.c_float))
     check(lib().hfx_serve_workload(C.byref(d), C.byref(r)))
     return WorkloadResult(r.wall_s, r.windows_retired, r.max_concurrent, r.admissions_deferred,
-                          r.ingest_deferred, lat, gpu, adm, fin, cap)
+                          r.ingest_deferred, lat, gpu, wait, adm, fin, cap, r.windows_ingested_late,
+                          r.clock_error_us)
 
 
-def summarize(specs: Sequence[StreamSpec], res: WorkloadResult):
+def summarize(specs: Sequence[StreamSpec], res: WorkloadResult, periods_s=None):
+    """Throughput and latency percentiles of a workload run; `periods_s` are the streams' window
+    periods at the rates they ran at (default: their cameras' own stride / f_s)."""
     lat = res.latency_us
-    periods_us = np.array([s.window_period_s * 1e6 for s in specs])[:, None]
+    if periods_s is None:
+        periods_s = [s.window_period_s for s in specs]
+    periods_us = np.asarray(periods_s, np.float64)[:, None] * 1e6
     ratio = (lat / periods_us).ravel()
     pix = np.array([s.height * s.width for s in specs], np.float64)
     out = {
@@ -195,11 +209,16 @@ def summarize(specs: Sequence[StreamSpec], res: WorkloadResult):
         "latency_us_p50": round(float(np.percentile(lat, 50)), 1),
         "latency_us_p99": round(float(np.percentile(lat, 99)), 1),
         "gpu_us_p50": round(float(np.percentile(res.gpu_us, 50)), 1),
+        "wait_us_p50": round(float(np.percentile(res.wait_us, 50)), 1),
+        "wait_us_p99": round(float(np.percentile(res.wait_us, 99)), 1),
+        "latency_over_window_period_p50": round(float(np.percentile(ratio, 50)), 4),
         "latency_over_window_period_p99": round(float(np.percentile(ratio, 99)), 4),
         "frac_windows_within_period": round(float((ratio < 1.0).mean()), 4),
         "max_concurrent_streams": int(res.max_concurrent),
         "admissions_deferred": int(res.admissions_deferred),
         "ingest_deferred": int(res.ingest_deferred),
+        "windows_ingested_late": int(res.windows_ingested_late),
+        "clock_error_us": round(float(res.clock_error_us), 1),
     }
     return out
 
@@ -222,9 +241,19 @@ class Server:
         check(rc)
         return sid.value
 
-    def ingest(self, sid: int, n_frames: int) -> bool:
-        """False when the ring is full (windows not retired yet)."""
-        rc = lib().hfx_server_ingest(self._h, sid, n_frames)
+    def clock(self) -> float:
+        """Seconds on the server's clock."""
+        t = C.c_double()
+        check(lib().hfx_server_clock(self._h, C.byref(t)))
+        return t.value
+
+    def ingest(self, sid: int, n_frames: int, due_s: Optional[float] = None) -> bool:
+        """False when the ring is full (windows not retired yet).  due_s: server-clock time the
+        camera delivered the last of these frames (default: now)."""
+        if due_s is None:
+            rc = lib().hfx_server_ingest(self._h, sid, n_frames)
+        else:
+            rc = lib().hfx_server_ingest_due(self._h, sid, n_frames, float(due_s))
         if rc == _lib.HFX_NOT_READY:
             return False
         check(rc)
@@ -239,7 +268,8 @@ class Server:
         buf = (_lib.WindowDone * cap)()
         n = C.c_int32(0)
         check(lib().hfx_server_poll(self._h, buf, cap, C.byref(n)))
-        return [(buf[i].stream_id, buf[i].window, buf[i].latency_us, buf[i].gpu_us) for i in range(n.value)]
+        return [(buf[i].stream_id, buf[i].window, buf[i].latency_us, buf[i].gpu_us, buf[i].wait_us)
+                for i in range(n.value)]
 
     def window_output(self, sid: int, window: int, shape) -> np.ndarray:
         """Copies a retired window's moments ([n_out][OH][OW] float32) to the host."""

@@ -250,6 +250,14 @@ hfx_status hfx_convert(const void *in, int32_t in_dtype, void *out, int32_t out_
 hfx_status hfx_slide_avg_step(const float *frame, float *old_slot, float *running, int64_t n, int32_t window,
                               hfx_stream_t stream);
 
+/* HBM bandwidth probe (measurement only; no counterpart in the reference): mode 0 reads
+ * `bytes` of `in`, 1 writes `bytes` of `out`, 2 copies `in` to `out`, with 16-byte vectors,
+ * `unroll` (1, 2, 4, 8) independent vectors per lane in flight, blocks_per_cu x CUs
+ * workgroups of 256 lanes, non-temporal accesses when `nontemporal` is set.  Buffers 16-byte
+ * aligned; mode 0 uses the first KiB of `out` as a sink (its contents become undefined). */
+hfx_status hfx_hbm_probe(int32_t mode, const void *in, void *out, int64_t bytes, int32_t blocks_per_cu,
+                         int32_t unroll, int32_t nontemporal, hfx_stream_t stream);
+
 /* ------------------------------------------------------------------------------------
  * Seeded synthetic interferograms (replaces the camera / holofile sources for tests and
  * benchmarks; no reference counterpart on the path).  phase_state: device array of

@@ -14,8 +14,12 @@
  * (launch every ready window of every stream) -> poll (retire finished windows in order,
  * with their frame-to-moment latency) -> release.
  *
- * Latency of a window = time from the ingest event recorded after its last frame was
- * written to the completion event of its moments (HIP events, same device clock).
+ * Latency of a window = time from its due time (the camera time of its last frame, given to
+ * hfx_server_ingest_due, or the host time of the hfx_server_ingest call) to the completion
+ * of its moments on the GPU.  GPU completion times are read on the server's host clock
+ * through an epoch event recorded at hfx_server_create; they are late by at most
+ * hfx_server_info.clock_error_us, never early, so a latency is never under-reported.  Time a
+ * window waits for ring space before it can be ingested counts, since its due time is kept.
  *
  * One host thread drives a server; calls on one server must not overlap.
  */
@@ -47,9 +51,10 @@ typedef struct hfx_server_desc {
 } hfx_server_desc;
 
 /* Launch policy of hfx_server_schedule.
- * FIFO: every ready window goes at once to its stream's lane, in arrival order per lane.
+ * FIFO: every ready window goes at once to its stream's lane, oldest due time first across
+ *       the streams (a stream's windows keep their order on its lane).
  * EDF: ready windows wait on the host; whenever a lane holds fewer than lane_depth windows,
- *      the ready window with the earliest deadline (ready time + the stream's window period
+ *      the ready window with the earliest deadline (due time + the stream's window period
  *      stride / fs_hz) of a stream with no window on the GPU is launched there.  A stream has
  *      at most one window in flight (its plan workspace), so its windows keep their order.
  * FIFO is the default: under the mixed 70 %-load workload EDF kept the same share of windows
@@ -66,8 +71,9 @@ typedef struct hfx_window_done {
   int32_t stream_id;
   int32_t n_outputs;         /* images per window (3 or 5) */
   int64_t window;            /* window index within the stream (0, 1, ...) */
-  float latency_us;          /* last frame ingested -> moments complete */
+  float latency_us;          /* due time (last frame delivered) -> moments complete */
   float gpu_us;              /* launch -> complete on its lane */
+  float wait_us;             /* due time -> start of its work on its lane */
 } hfx_window_done;
 
 typedef struct hfx_server_info {
@@ -77,6 +83,7 @@ typedef struct hfx_server_info {
   int64_t bytes_free;        /* hipMemGetInfo free bytes */
   int64_t windows_launched;
   int64_t windows_retired;
+  double clock_error_us;     /* bound on how late GPU times read on the server clock are */
 } hfx_server_info;
 
 void hfx_server_desc_init(hfx_server_desc *d);
@@ -89,9 +96,15 @@ hfx_status hfx_server_create(const hfx_server_desc *d, hfx_server **out);
 hfx_status hfx_server_destroy(hfx_server *s);
 /* HFX_NOT_READY when no slot is free or the stream does not fit the memory budget now. */
 hfx_status hfx_server_admit(hfx_server *s, const hfx_stream_desc *d, int32_t *stream_id);
+/* Seconds on the server's clock (host steady clock, zero at hfx_server_create). */
+hfx_status hfx_server_clock(hfx_server *s, double *t_s);
 /* The camera writes the next n_frames frames of the stream into its ring; HFX_NOT_READY
- * (nothing written) while that would overwrite frames of a window not yet retired. */
+ * (nothing written) while that would overwrite frames of a window not yet retired.
+ * hfx_server_ingest_due stamps the windows these frames complete with due_s (server clock):
+ * the time the camera delivered the last of them; hfx_server_ingest stamps them with the time
+ * of the call. */
 hfx_status hfx_server_ingest(hfx_server *s, int32_t stream_id, int64_t n_frames);
+hfx_status hfx_server_ingest_due(hfx_server *s, int32_t stream_id, int64_t n_frames, double due_s);
 /* Launches every window whose frames are in (all streams); *launched = count. */
 hfx_status hfx_server_schedule(hfx_server *s, int32_t *launched);
 /* Retires finished windows of all streams, oldest first per stream: up to cap records. */
@@ -104,12 +117,17 @@ hfx_status hfx_server_release(hfx_server *s, int32_t stream_id);
 hfx_status hfx_server_info_get(hfx_server *s, hfx_server_info *info);
 
 /* ------------------------------------------------------------------------------------
- * Native workload driver: Poisson arrivals of streams and of their windows.
- * Stream i arrives at arrival_s[i] (seconds from start), waits until admitted, then its
- * windows arrive as a Poisson process of rate window_rate_hz[i] (the first window needs
- * T frames, every later one S new frames); after windows_per_stream windows are retired
- * the stream is released.  The loop admits, ingests, schedules and polls continuously.
+ * Native workload driver: streams arrive at given times; their cameras deliver windows.
+ * Stream i arrives at arrival_s[i] (seconds from start) and waits until admitted; then its
+ * camera delivers windows at window_rate_hz[i]: PERIODIC, one stride of frames every
+ * 1/rate (window 0 due T/S strides after admission), as a camera running at
+ * rate * stride frames/s does; POISSON, exponential gaps of mean 1/rate.  Each window is
+ * ingested at or after its due time (later while the ring is full) and its latency counts
+ * from the due time.  After windows_per_stream windows are retired the stream is released.
+ * The loop admits, ingests, schedules and polls continuously on the calling thread.
  * ---------------------------------------------------------------------------------- */
+enum hfx_arrivals { HFX_ARRIVALS_PERIODIC = 0, HFX_ARRIVALS_POISSON = 1 };
+
 typedef struct hfx_workload_desc {
   hfx_server_desc server;
   int32_t n_streams;
@@ -121,6 +139,7 @@ typedef struct hfx_workload_desc {
   int32_t capture_stream;          /* index whose window outputs are copied to capture_out, or -1 */
   float *capture_out;              /* host, windows_per_stream * n_out * OH * OW floats */
   double time_limit_s;             /* the loop stops (HFX_E_HIP) if exceeded */
+  int32_t arrivals;                /* enum hfx_arrivals: window arrival process */
 } hfx_workload_desc;
 
 typedef struct hfx_workload_result {
@@ -133,6 +152,9 @@ typedef struct hfx_workload_result {
   float *gpu_us;                   /* same shape (or NULL) */
   double *admit_s;                 /* caller array n_streams (or NULL): admission times */
   double *finish_s;                /* caller array n_streams (or NULL): release times */
+  float *wait_us;                  /* caller array n_streams * windows_per_stream (or NULL) */
+  int64_t windows_ingested_late;   /* windows ingested more than one period after their due time */
+  double clock_error_us;           /* the server's GPU-time bound (hfx_server_info.clock_error_us) */
 } hfx_workload_result;
 
 hfx_status hfx_serve_workload(const hfx_workload_desc *d, hfx_workload_result *r);

@@ -115,23 +115,27 @@ def test_step_holds_a_popped_window_when_a_later_queue_lags(gpu, tmp_path):
     """Once mode (step()): a section fed by two queues pops the first, finds the second not
     ready and is skipped; the first queue's window must stay in the queue (held, not released:
     a release advances the read index and would drop it), so the section later runs on the
-    windows the two queues hand out in order.  Q1 hands out 1 frame per run and Q2 2 frames, so
-    run i multiplies frame i by frames 2i, 2i+1 (products of 12-bit values are exact in F32)."""
+    windows the two queues hand out in order.  Each queue has its own source reading the same
+    recording (so a full Q1 stalls only its own source); Q1 hands out 1 frame per run and Q2 2
+    frames, so run i multiplies frame i by frames 2i, 2i+1 (products of 12-bit values are exact
+    in F32).  Q1 is ready one step before Q2 on every run."""
     rng = np.random.default_rng(5)
     frames = rng.integers(0, 4096, size=(16, 8, 16)).astype(np.uint16)
     p = tmp_path / "lag.holo"
     p.write_bytes(OH.pack(frames, {"version": 1}))
     spec = {"nodes": {
-        "source": {"type": "Holofile", "params": {"path": str(p), "load_kind": "GPUCached", "start_frame": 0,
-                                                  "end_frame": 16, "batch_size": 1, "keep_cursor": False}},
+        "source_a": {"type": "Holofile", "params": {"path": str(p), "load_kind": "GPUCached", "start_frame": 0,
+                                                    "end_frame": 16, "batch_size": 1, "keep_cursor": False}},
+        "source_b": {"type": "Holofile", "params": {"path": str(p), "load_kind": "GPUCached", "start_frame": 0,
+                                                    "end_frame": 16, "batch_size": 1, "keep_cursor": False}},
         "conv_a": {"type": "Conversion", "params": {"target": "F32", "strategy": "Real"}},
         "conv_b": {"type": "Conversion", "params": {"target": "F32", "strategy": "Real"}},
         "q1": {"type": "BatchQueue", "params": {"target_capacity": 4, "output_size": 1, "output_stride": 1}},
         "q2": {"type": "BatchQueue", "params": {"target_capacity": 4, "output_size": 2, "output_stride": 2}},
         "product": {"type": "Multiply", "params": {}},
     }, "edges": [
-        {"from": "source", "to": "conv_a", "out": 0, "in": 0},
-        {"from": "source", "to": "conv_b", "out": 0, "in": 0},
+        {"from": "source_a", "to": "conv_a", "out": 0, "in": 0},
+        {"from": "source_b", "to": "conv_b", "out": 0, "in": 0},
         {"from": "conv_a", "to": "q1", "out": 0, "in": 0},
         {"from": "conv_b", "to": "q2", "out": 0, "in": 0},
         {"from": "q1", "to": "product", "out": 0, "in": 0},
@@ -149,5 +153,5 @@ def test_step_holds_a_popped_window_when_a_later_queue_lags(gpu, tmp_path):
         exp = f[runs][None] * f[2 * runs:2 * runs + 2]
         assert np.array_equal(out, exp), f"run {runs} (step {step})"
         runs += 1
-    assert runs >= 4
+    assert runs >= 5   # one run every second step
     g.close()

@@ -104,7 +104,7 @@ def test_server_api_backpressure_and_outputs(gpu):
         done += srv.poll()
         if len(done) == 5:
             break
-    assert [w for _, w, _, _ in done] == [0, 1, 2, 3, 4]
+    assert [w for _, w, _, _, _ in done] == [0, 1, 2, 3, 4]
     assert srv.ingest(sid, 8)                       # retired windows free the ring
     out = srv.window_output(sid, 4, (3, 512, 512))
     ref = _eager(spec, 5)[4]
@@ -134,3 +134,31 @@ def test_full_mix_alone_and_in_mix_identical(gpu, k):
                                  server=serving.server_desc(max_streams=1, n_lanes=1), time_limit_s=90)
     assert np.array_equal(alone.capture, mix.capture)
     assert np.array_equal(alone.capture, _eager(specs[k], wps))
+
+
+@pytest.mark.gpu
+def test_latency_counts_from_the_camera_due_time(gpu):
+    """A window whose frames the camera delivered 10 ms before they were ingested (as when the
+    ring was full) reports a latency from that due time: it includes the 10 ms, and latency =
+    wait before its work starts + its GPU time, all read on the server's clock."""
+    spec = serving.StreamSpec(512, 512, 32, 8, 5, 9, 0.3, 37e3, seed=3, stream_id=0)
+    srv = serving.Server(serving.server_desc(max_streams=1, n_lanes=1))
+    sid = srv.admit(spec)
+    t = srv.clock()
+    assert srv.ingest(sid, 32, due_s=t - 0.010)
+    assert srv.schedule() == 1
+    done = []
+    for _ in range(100000):
+        done += srv.poll()
+        if done:
+            break
+    (_, w, lat, gpu_us, wait), = done
+    assert w == 0
+    assert wait >= 10_000.0
+    assert lat >= 10_000.0 + gpu_us
+    assert abs((lat - wait) - gpu_us) < 5.0
+    info = srv.info()
+    assert 0.0 < info["clock_error_us"] < 10_000.0
+    srv.release(sid)
+    srv.close()
+

@@ -31,3 +31,19 @@ def test_stream_bytes_rejects_bad_geometry():
     s = serving.StreamSpec(512, 512, 32, 12, 6, 13, 0.3, 37e3, seed=1, stream_id=0)   # T % S != 0
     with pytest.raises(HfxInvalidArgument):
         serving.stream_bytes(s)
+
+
+def test_summarize_uses_the_given_periods():
+    import numpy as np
+    specs = [serving.StreamSpec(512, 512, 32, 8, 5, 9, 0.3, 40e3, seed=1, stream_id=0),
+             serving.StreamSpec(512, 512, 32, 8, 5, 9, 0.3, 20e3, seed=2, stream_id=1)]
+    lat = np.array([[100.0, 300.0], [100.0, 500.0]], np.float32)
+    res = serving.WorkloadResult(1.0, 4, 2, 0, 0, lat, lat / 2, lat / 4, np.zeros(2), np.ones(2), None)
+    # camera periods: 200 us and 400 us -> ratios 0.5, 1.5, 0.25, 1.25
+    out = serving.summarize(specs, res)
+    assert out["frac_windows_within_period"] == 0.5
+    # periods at a slower rate (1 ms each): every window within
+    out = serving.summarize(specs, res, periods_s=[1e-3, 1e-3])
+    assert out["frac_windows_within_period"] == 1.0
+    assert out["latency_over_window_period_p99"] < 0.5
+
