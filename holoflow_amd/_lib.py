@@ -150,6 +150,8 @@ SIGNATURES = {
     "hfx_convert_u16": (C.c_int, [_VP, _VP, C.c_int64, C.c_int32, _VP]),
     "hfx_convert_scratch_bytes": (C.c_int64, [C.c_int64]),
     "hfx_convert": (C.c_int, [_VP, C.c_int32, _VP, C.c_int32, C.c_int32, C.c_int64, _VP, _VP]),
+    "hfx_convert_noalloc_scratch_bytes": (C.c_int64, []),
+    "hfx_convert_noalloc": (C.c_int, [_VP, C.c_int32, _VP, C.c_int32, C.c_int32, C.c_int64, _VP]),
     "hfx_hbm_probe": (C.c_int, [C.c_int32, _VP, _VP, C.c_int64, C.c_int32, C.c_int32, C.c_int32, _VP]),
     "hfx_slide_avg_step": (C.c_int, [_VP, _VP, _VP, C.c_int64, C.c_int32, _VP]),
     "hfx_synth_desc_init": (None, [_P(SynthDesc), C.c_int32, C.c_int32]),

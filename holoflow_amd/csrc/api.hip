@@ -1023,6 +1023,23 @@ hfx_status hfx_hbm_probe(int32_t mode, const void *in, void *out, int64_t bytes,
   return HFX_OK;
 }
 
+int64_t hfx_convert_noalloc_scratch_bytes(void) { return hfx::convert_static_scratch_bytes(); }
+
+hfx_status hfx_convert_noalloc(const void *in, int32_t in_dtype, void *out, int32_t out_dtype, int32_t strategy,
+                               int64_t n, hfx_stream_t stream) {
+  if (n < 0) return fail(HFX_E_INVALID, "negative element count");
+  void *scratch = nullptr;
+  if (strategy == HFX_CONV_SCALED && n > 0) {
+    if (hfx_convert_scratch_bytes(n) > hfx::convert_static_scratch_bytes())
+      return fail(HFX_E_INVALID, "hfx_convert_noalloc: " + std::to_string(n) + " elements need " +
+                                     std::to_string(hfx_convert_scratch_bytes(n)) + " scratch bytes, more than the " +
+                                     std::to_string(hfx::convert_static_scratch_bytes()) + " the library holds");
+    if (hfx::convert_static_scratch(&scratch) != 0)
+      return fail(HFX_E_HIP, "hfx_convert_noalloc: device address of the library scratch not available");
+  }
+  return hfx_convert(in, in_dtype, out, out_dtype, strategy, n, scratch, stream);
+}
+
 hfx_status hfx_slide_avg_step(const float *frame, float *old_slot, float *running, int64_t n, int32_t window,
                               hfx_stream_t stream) {
   if ((!frame || !old_slot || !running) && n > 0) return fail(HFX_E_INVALID, "null argument");

@@ -118,6 +118,8 @@ int launch_multiply(const void *a, int a_complex, const long long *sa, const voi
                     const long long *sb, int ndim, const long long *shape, void *out, hipStream_t st);
 int launch_convert_u16(const uint16_t *in, void *out, long long n, int to_complex, hipStream_t st);
 int minmax_blocks(long long n);
+long long convert_static_scratch_bytes();
+int convert_static_scratch(void **ptr);
 int launch_convert(const void *in, int in_dt, void *out, int out_dt, int strategy, long long n,
                    void *scratch, hipStream_t st);
 int launch_hbm_probe(int mode, const void *in, void *out, long long bytes, int blocks_per_cu, int unroll,

@@ -5,6 +5,8 @@
 //               optional f_k / f_k^2 weights (mean_abs.cu:239-276, mean.cu:268-306,
 //               multiply.cu:69-92 wired by graph_builder.cc:612-631)
 //   convert   : uint16 -> float32 or complex64 (x + 0i) (conversion.cu:156-202)
+#include <mutex>
+
 #include "hfx_kernels.hh"
 
 namespace hfx {
@@ -207,6 +209,34 @@ __global__ void k_scaled(const float *in, void *out, int out_dt, long long n, co
 }
 
 int minmax_blocks(long long n) { return (int)((n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024); }
+
+// Library-held scratch of the allocation-free Scaled conversion: the per-block min/max
+// partials of at most 1024 blocks plus the folded pair (8,200 bytes, any n).  Each GPU holds
+// its own copy for the life of the process; calls on one GPU share it, so calls that overlap
+// on one GPU (concurrent streams) race.  Every call writes each pair it reads first, so
+// nothing needs clearing between calls.
+constexpr int kConvScratchPairs = 1025;
+static __device__ __align__(256) float2 scratch[kConvScratchPairs];
+
+long long convert_static_scratch_bytes() { return (long long)sizeof(float2) * kConvScratchPairs; }
+
+// Device address of `scratch` on the current device, looked up once per device (the host
+// symbol is only a stand-in: kernels must receive this address).
+int convert_static_scratch(void **ptr) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static void *per_device[kMaxDevices] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return -1;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!per_device[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -1;
+    per_device[dev] = p;
+  }
+  *ptr = per_device[dev];
+  return 0;
+}
 
 int launch_convert(const void *in, int in_dt, void *out, int out_dt, int strategy, long long n,
                    void *scratch, hipStream_t st) {

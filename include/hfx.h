@@ -245,6 +245,17 @@ int64_t hfx_convert_scratch_bytes(int64_t n);
 hfx_status hfx_convert(const void *in, int32_t in_dtype, void *out, int32_t out_dtype, int32_t strategy,
                        int64_t n, void *scratch, hfx_stream_t stream);
 
+/* hfx_convert without the scratch argument, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The Scaled min/max partials live in a library-held
+ * per-device array of hfx_convert_noalloc_scratch_bytes() bytes (8,200: enough for any n);
+ * the call allocates nothing and rejects an n whose partials would not fit with
+ * HFX_E_INVALID.  Calls on one GPU share that array: calls that overlap on one GPU (on
+ * concurrent streams) race. */
+int64_t hfx_convert_noalloc_scratch_bytes(void);
+hfx_status hfx_convert_noalloc(const void *in, int32_t in_dtype, void *out, int32_t out_dtype, int32_t strategy,
+                               int64_t n, hfx_stream_t stream);
+
 /* One push of holotask::asyncs::SlidingAverage (slide_avg.cu:102-125, 200-215):
  * running = (running + frame / window) - old_slot / window, then old_slot = running. */
 hfx_status hfx_slide_avg_step(const float *frame, float *old_slot, float *running, int64_t n, int32_t window,
