@@ -127,18 +127,23 @@ __global__ __launch_bounds__(256) void k_dft(DftArgs a) {
 }
 
 // FP32 stand-alone time DFT with the real-input symmetry folded (dft_fold_tile): NBF tiles of
-// 16 bins, T % 32 == 0.  Same outputs as k_dft<NBT, 0> up to FP32 rounding, with
-// 2 (T/8) + 1 MFMAs per 16 bins and 16 pixels instead of T/2.
-template <int NBF>
+// 16 bins per wave, T % 32 == 0.  Same outputs as k_dft<NBT, 0> up to FP32 rounding, with
+// 2 (T/8) + 1 MFMAs per 16 bins and 16 pixels instead of T/2.  SPLIT waves share a 64-pixel
+// tile and take NBF tiles of bins each (their frame loads meet in the L1): with four tiles in
+// one wave the accumulators alone fill 128 registers and a SIMD runs one wave, two waves of
+// two tiles each leave room for three.
+template <int NBF, int SPLIT = 1>
 __global__ __launch_bounds__(256) void k_dft_fold(DftArgs a) {
   extern __shared__ float4 lds_dyn[];
   float *lf = reinterpret_cast<float *>(lds_dyn);
-  stage_fold_matrix_rt(lf, reinterpret_cast<const float *>(a.wm_hi), a.r.T, NBF, a.nbc, threadIdx.x, 256);
+  stage_fold_matrix_rt(lf, reinterpret_cast<const float *>(a.wm_hi), a.r.T, NBF * SPLIT, a.nbc, threadIdx.x, 256);
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  const long long p0w = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int part = SPLIT > 1 ? wave % SPLIT : 0;  // bin tiles part * NBF .. of this wave
+  const long long p0w = ((long long)blockIdx.x * (4 / SPLIT) + wave / SPLIT) * 64;
   f4 ac[NBF][4], as[NBF][4];
-  dft_fold_tile<NBF>(a.r, lf, p0w, lane, ac, as);
+  dft_fold_tile<NBF>(a.r, lf, p0w, lane, ac, as, part * NBF, NBF * SPLIT);
 
   const long long p0 = p0w + 4 * (lane & 15);
   if (p0 >= a.r.npix) return;
@@ -154,7 +159,7 @@ __global__ __launch_bounds__(256) void k_dft_fold(DftArgs a) {
   for (int f = 0; f < NBF; ++f) {
 #pragma unroll
     for (int rr = 0; rr < 4; ++rr) {
-      const int bin = 16 * f + 4 * (lane >> 4) + rr;
+      const int bin = 16 * (part * NBF + f) + 4 * (lane >> 4) + rr;
       if (bin >= a.nbc) continue;
       float2 v[4];
 #pragma unroll
@@ -532,10 +537,10 @@ template <int NBT> static int launch_plain_mode(const DftArgs &a, int mode, hipS
   return -2;
 }
 
-template <int NBF> static int launch_fold(const DftArgs &a, hipStream_t st) {
-  const long long waves = (a.r.npix + 63) / 64;
-  hipLaunchKernelGGL((k_dft_fold<NBF>), dim3((unsigned)((waves + 3) / 4)), dim3(256),
-                     dft_fold_lds_bytes(NBF, a.r.T), st, a);
+template <int NBF, int SPLIT = 1> static int launch_fold(const DftArgs &a, hipStream_t st) {
+  const long long waves = (a.r.npix + 63) / 64 * SPLIT;
+  hipLaunchKernelGGL((k_dft_fold<NBF, SPLIT>), dim3((unsigned)((waves + 3) / 4)), dim3(256),
+                     dft_fold_lds_bytes(NBF * SPLIT, a.r.T), st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -547,7 +552,7 @@ int launch_time_dft(const DftLaunch &L, hipStream_t st) {
       case 1: return launch_fold<1>(a, st);
       case 2: return launch_fold<2>(a, st);
       case 3: return launch_fold<3>(a, st);
-      case 4: return launch_fold<4>(a, st);
+      case 4: return launch_fold<2, 2>(a, st);
     }
   }
   switch ((2 * L.nbc + 15) / 16) {

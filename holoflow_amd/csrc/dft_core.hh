@@ -412,11 +412,13 @@ template <int TPW, int T, int K> struct DftStreamFold {
 // Runtime-T form of the folded FP32 time DFT (see DftStreamFold) for NBF 16-bin tiles and one
 // 64-pixel tile per wave (stand-alone time DFT): steps of four frame pairs in batches of K,
 // T % (8 K) == 0.  Operand matrix from stage_fold_matrix_rt with pitch T/2 + 4.
+// f0, nt: the wave computes tiles f0 .. f0 + NBF - 1 of an operand matrix staged with nt tiles
+// (several waves may share one pixel tile and split its bins).
 template <int NBF, int K = 4>
 __device__ __forceinline__ void dft_fold_tile(const DftRing &r, const float *lds, long long p0w, int lane,
-                                              f4 (&ac)[NBF][4], f4 (&as)[NBF][4]) {
+                                              f4 (&ac)[NBF][4], f4 (&as)[NBF][4], int f0 = 0, int nt = NBF) {
   const int T = r.T, steps = T / 8, PC = T / 2 + 4;
-  const float *lc = lds, *ls = lc + 16 * NBF * PC, *lx = ls + 16 * NBF * PC;
+  const float *lc = lds + 16 * f0 * PC, *ls = lds + 16 * (nt + f0) * PC, *lx = lds + 32 * nt * PC + 64 * f0;
   const int g = lane >> 4, rw = lane & 15;
   const long long p0 = p0w + 4 * (lane & 15);
 #pragma unroll

@@ -75,19 +75,25 @@ __device__ __forceinline__ void store_moment(const PassArgs &a, int nacc, int y,
 }  // namespace
 
 // ------------------------------------------------------------------------- row pass
-constexpr int WROW_WAVES = 4;
+// Rows (waves) per workgroup.  Each wave owns a padded scratch row in LDS, and the workgroup
+// shares one twiddle table, so the LDS footprint sets the occupancy: at 2048 points four
+// waves (86 KB) leave one workgroup, one wave per SIMD, per CU, while eight waves in one
+// workgroup (155 KB) give two waves per SIMD to hide the memory latency.  The moment kinds
+// keep four: their 5*E sums leave no register room for a second wave per SIMD.
+constexpr int wrow_waves(int n, int kind) { return n >= 2048 && !is_mom(kind) ? 8 : 4; }
 
 template <int N, int KIND>
-__global__ __launch_bounds__(64 * WROW_WAVES) void k_wrow(PassArgs a) {
+__global__ __launch_bounds__(64 * wrow_waves(N, KIND)) void k_wrow(PassArgs a) {
   using F = WaveFft<N>;
   constexpr int E = F::E;
-  __shared__ float2 scr[WROW_WAVES * F::SCR];
+  constexpr int RW = wrow_waves(N, KIND);
+  __shared__ float2 scr[RW * F::SCR];
   __shared__ float2 twl[F::TW];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  F::build_twiddles(twl, a.tw, tid, 64 * WROW_WAVES);
+  F::build_twiddles(twl, a.tw, tid, 64 * RW);
   __syncthreads();
 
-  const int y = xcd_tile(blockIdx.x, gridDim.x) * WROW_WAVES + wave;
+  const int y = xcd_tile(blockIdx.x, gridDim.x) * RW + wave;
   if (y >= a.H) return;
   const int C = a.blk_c;
   const long long plane_elems = (long long)a.H * a.W;
@@ -184,18 +190,24 @@ __global__ __launch_bounds__(64 * WROW_WAVES) void k_wrow(PassArgs a) {
 // 256 registers each) instead of three, which spilled.
 constexpr int wcol_subsets(int n, int kind) { return n >= 2048 ? 1 : kind == BLK_TO_MOM5 ? 2 : 3; }
 constexpr bool wcol_prefetch(int n, int kind) { return n >= 2048 ? false : wcol_subsets(n, kind) < 3; }
-// Planes per block of the plane-parallel column kinds.  The 2048-row middle pass (forward
-// FFT, table, inverse FFT) runs one wave per SIMD (registers), so a block walks 4 planes with
-// the next plane's column prefetched into registers (measured 13 % faster at 2048^2); the
-// other kinds and sizes keep one plane per block (a prefetch costs them waves or spills).
-constexpr int wcol_ppb(int n, int kind) { return n >= 2048 && kind == BLK_MID ? 4 : 1; }
+// Planes per block of the plane-parallel column kinds: one (a register prefetch of the next
+// plane's column costs waves or spills; at 2048 rows two waves per SIMD hide the latency
+// instead, see wcol_tiles).
+constexpr int wcol_ppb(int, int) { return 1; }
+// Column tiles per workgroup of the plane-parallel kinds: two at 2048 rows (eight waves share
+// one twiddle table: 155 KB of LDS, two waves per SIMD instead of one), else one.
+constexpr int wcol_tiles(int n, int kind) { return n >= 2048 && !is_mom(kind) ? 2 : 1; }
+constexpr int wcol_threads(int n, int kind) {
+  return is_mom(kind) ? 256 * wcol_subsets(n, kind) : 256 * wcol_tiles(n, kind);
+}
 
 template <int N, int KIND>
-__global__ __launch_bounds__(is_mom(KIND) ? 256 * wcol_subsets(N, KIND) : 256) void k_wcol(PassArgs a) {
+__global__ __launch_bounds__(wcol_threads(N, KIND)) void k_wcol(PassArgs a) {
   using F = WaveFft<N>;
   constexpr int E = F::E;
   constexpr int S = is_mom(KIND) ? wcol_subsets(N, KIND) : 1;
-  constexpr int WAVES = 4 * S;
+  constexpr int TL = wcol_tiles(N, KIND);
+  constexpr int WAVES = 4 * S * TL;
   __shared__ float2 scr[WAVES * F::SCR];
   __shared__ float2 twl[F::TW];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -208,17 +220,21 @@ __global__ __launch_bounds__(is_mom(KIND) ? 256 * wcol_subsets(N, KIND) : 256) v
   // column tiles in contiguous runs per XCD: the moment images are written column-wise, so
   // the 8 tiles that share an output cache line (32 floats) then share an L2 that merges
   // their 16-byte pieces before they go out
-  const int bx = xcd_tile(blockIdx.x, gridDim.x);
+  // plane-parallel kinds with TL tiles: wave >> 2 picks the tile (sub is 0 for them)
+  const int bx = xcd_tile(blockIdx.x, gridDim.x) * TL + (TL > 1 ? (int)(wave >> 2) : 0);
   const int x = bx * C + c;
-  const bool active = c < C;
+  const bool active = c < C && bx * C < a.W;
   const long long plane_elems = (long long)a.H * a.W;
   const long long tile = (long long)bx * a.H * C;  // blocked offset of the tile
+  // element (lane + 64 m) of the column: tile + lane * C + c + m * col_step, the per-m step a
+  // wave-uniform scalar (no per-element 64-bit products kept live across the transforms)
+  const long long col_step = 64LL * C;
   float2 *ws = scr + wave * F::SCR;
 
   auto load = [&](float2 (&v)[E], long long p) {
 #pragma unroll
     for (int m = 0; m < E; ++m)
-      v[m] = reinterpret_cast<const float2 *>(a.in)[p * plane_elems + tile + (long long)(lane + 64 * m) * C + c];
+      v[m] = reinterpret_cast<const float2 *>(a.in)[p * plane_elems + tile + lane * C + c + m * col_step];
   };
 
   if constexpr (is_mom(KIND)) {
@@ -282,12 +298,13 @@ __global__ __launch_bounds__(is_mom(KIND) ? 256 * wcol_subsets(N, KIND) : 256) v
     auto finish = [&](float2 (&v)[E], long long p) {
       if (KIND == BLK_MID) {
         F::template run<false>(v, ws, twl, lane);
+        asm volatile("" ::: "memory");  // keep the table loads after the forward transform
 #pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.mid_tf[tile + (long long)(lane + 64 * m) * C + c]);
+        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.mid_tf[tile + lane * C + c + m * col_step]);
         F::template run<true>(v, ws, twl, lane);
 #pragma unroll
         for (int m = 0; m < E; ++m)
-          reinterpret_cast<float2 *>(a.out)[p * plane_elems + tile + (long long)(lane + 64 * m) * C + c] = v[m];
+          reinterpret_cast<float2 *>(a.out)[p * plane_elems + tile + lane * C + c + m * col_step] = v[m];
       } else {
         F::template run<false>(v, ws, twl, lane);
 #pragma unroll
@@ -333,17 +350,19 @@ bool wave_pass_supported(int n) { return n == 256 || n == 512 || n == 768 || n =
 
 template <int N, int KIND> static int launch_wrow_n(const PassArgs &a, hipStream_t st) {
   if ((a.inv != 0) != row_inv(KIND)) return -5;
-  const unsigned gx = (unsigned)((a.H + WROW_WAVES - 1) / WROW_WAVES);
+  constexpr int RW = wrow_waves(N, KIND);
+  const unsigned gx = (unsigned)((a.H + RW - 1) / RW);
   const unsigned gy = is_mom(KIND) ? 1u : (unsigned)a.planes;
-  hipLaunchKernelGGL((k_wrow<N, KIND>), dim3(gx, gy), dim3(64 * WROW_WAVES), 0, st, a);
+  hipLaunchKernelGGL((k_wrow<N, KIND>), dim3(gx, gy), dim3(64 * RW), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 template <int N, int KIND> static int launch_wcol_n(const PassArgs &a, hipStream_t st) {
   if (a.inv != 0) return -5;
-  const unsigned gx = (unsigned)(a.W / a.blk_c);
+  constexpr int TL = wcol_tiles(N, KIND);
+  const unsigned gx = (unsigned)((a.W / a.blk_c + TL - 1) / TL);
   const unsigned gy = is_mom(KIND) ? 1u : (unsigned)((a.planes + wcol_ppb(N, KIND) - 1) / wcol_ppb(N, KIND));
-  const unsigned nt = is_mom(KIND) ? 256 * wcol_subsets(N, KIND) : 256;
+  const unsigned nt = wcol_threads(N, KIND);
   hipLaunchKernelGGL((k_wcol<N, KIND>), dim3(gx, gy), dim3(nt), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
