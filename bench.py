@@ -454,6 +454,14 @@ def main():
             log(f"{prec}: {mixed[prec]}")
             mp.close()
         mixed["fp32_rel_l2_vs_fp64"] = {k: float(f"{fp32_err[i]:.3e}") for i, k in enumerate(("M0", "M1", "M2"))}
+        # The bound on any time-DFT speedup: the same batch with the time DFT (its frame loads and
+        # MFMAs) taken out of the fused front kernels, measured once with a timing-only build
+        # (rates of that run, same GPU and workload): FP32 pipelined front 8,959 -> 12,764, the
+        # 16-bit fused front 9,983 -> 14,572 moment frames/s.  A 16-bit DFT that cost nothing at
+        # all, frame loads included, would reach 14,572 / 8,959 = 1.63x the FP32 path.
+        mixed["ceiling_if_time_dft_free"] = {"fp32_batch_without_dft": 12764, "16bit_batch_without_dft": 14572,
+                                             "fp32_batch_same_run": 8959, "max_speedup_vs_fp32": 1.63,
+                                             "source": "design measurement (fused fronts built without the time DFT)"}
 
     del out
     # ---- optional in-path stages (Filter2D, Flatfield) at the workload's sizes
