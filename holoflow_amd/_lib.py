@@ -81,7 +81,7 @@ class PlanInfo(C.Structure):
         ("out_height", C.c_int32), ("out_width", C.c_int32),
         ("n_outputs", C.c_int32), ("n_planes", C.c_int32),
         ("workspace_bytes", C.c_int64),
-        ("batch_lanes", C.c_int32),
+        ("batch_lanes", C.c_int32), ("groupable", C.c_int32),
     ]
 
 
@@ -121,6 +121,7 @@ SIGNATURES = {
     "hfx_plan_update_z": (C.c_int, [_VP, C.c_double, _VP]),
     "hfx_moments": (C.c_int, [_VP, _VP, C.c_int64, C.c_int64, _VP, _VP]),
     "hfx_moments_batch": (C.c_int, [_VP, _VP, C.c_int64, C.c_int64, C.c_int32, C.c_int64, _VP, _VP]),
+    "hfx_moments_group": (C.c_int, [_P(_VP), _P(_VP), _P(C.c_int64), _P(C.c_int64), _P(_VP), C.c_int32, _VP]),
     "hfx_plan_time_dft": (C.c_int, [_VP, _VP, C.c_int64, C.c_int64, _VP, _VP]),
     "hfx_plan_run_stage": (C.c_int, [_VP, C.c_int32, _VP, C.c_int64, C.c_int64, _VP, _VP]),
     "hfx_dft_axis": (C.c_int, [_VP, C.c_int32, C.c_int64, C.c_int32, C.c_int64, C.c_int32, C.c_int32,
@@ -177,7 +178,7 @@ class ServerDesc(C.Structure):
                 ("policy", C.c_int32), ("lane_depth", C.c_int32)]
 
 
-SCHED_FIFO, SCHED_EDF = 0, 1
+SCHED_FIFO, SCHED_EDF, SCHED_GROUP = 0, 1, 2
 
 
 class StreamDesc(C.Structure):
@@ -192,7 +193,7 @@ class WindowDone(C.Structure):
 class ServerInfo(C.Structure):
     _fields_ = [("active_streams", C.c_int32), ("max_streams", C.c_int32), ("bytes_held", C.c_int64),
                 ("bytes_free", C.c_int64), ("windows_launched", C.c_int64), ("windows_retired", C.c_int64),
-                ("clock_error_us", C.c_double)]
+                ("clock_error_us", C.c_double), ("group_launches", C.c_int64), ("grouped_windows", C.c_int64)]
 
 
 class WorkloadDesc(C.Structure):
@@ -211,7 +212,7 @@ class WorkloadResult(C.Structure):
                 ("latency_us", C.POINTER(C.c_float)), ("gpu_us", C.POINTER(C.c_float)),
                 ("admit_s", C.POINTER(C.c_double)), ("finish_s", C.POINTER(C.c_double)),
                 ("wait_us", C.POINTER(C.c_float)), ("windows_ingested_late", C.c_int64),
-                ("clock_error_us", C.c_double)]
+                ("clock_error_us", C.c_double), ("group_launches", C.c_int64), ("grouped_windows", C.c_int64)]
 
 
 SIGNATURES.update({

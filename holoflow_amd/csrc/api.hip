@@ -341,6 +341,9 @@ hfx_status hfx_plan_get_info(const hfx_plan *p, hfx_plan_info *info) {
   info->n_planes = p->n_planes;
   info->workspace_bytes = (int64_t)(p->arena.bytes + p->f2d.arena.bytes);
   info->batch_lanes = p->lanes;
+  info->groupable = p->fused_row && !p->filter2d && !p->ff &&
+                    hfx::dft_front_pipelined(p->W, p->nbc, p->d.window, p->n_planes, p->d.precision) &&
+                    hfx::wave_pass_supported(p->OH) && hfx::wave_pass_supported(p->OW);
   return HFX_OK;
 }
 
@@ -494,6 +497,100 @@ static hfx_status window_moments(hfx_plan *p, const uint16_t *ring, int64_t ring
                                  int64_t first, float *out, hipStream_t st) {
   HFX_TRY(window_front(p, ring, ring_frames, first, st));
   return window_back(p, out, st);
+}
+
+// A group of plans whose windows can share one front launch and one launch per back pass:
+// the same frame and output geometry, window, band, weights and kernels; the propagation
+// tables (z, lambda, pitch), rings and outputs may differ.
+static bool group_compatible(const hfx_plan *a, const hfx_plan *b) {
+  const hfx_plan_desc &x = a->d, &y = b->d;
+  return a->H == b->H && a->W == b->W && a->OH == b->OH && a->OW == b->OW && x.window == y.window &&
+         x.prop == y.prop && x.precision == y.precision && x.flags == y.flags && x.bin_begin == y.bin_begin &&
+         x.bin_end == y.bin_end && x.split_bin == y.split_bin && x.weight_fs_hz == y.weight_fs_hz &&
+         x.pad_height == y.pad_height && x.pad_width == y.pad_width && a->n_planes == b->n_planes &&
+         a->n_out == b->n_out && a->blk_c == b->blk_c && a->fused_row == b->fused_row && a->filter2d == b->filter2d &&
+         (a->ff != nullptr) == (b->ff != nullptr);
+}
+
+hfx_status hfx_moments_group(hfx_plan *const *plans, const uint16_t *const *rings, const int64_t *ring_frames,
+                             const int64_t *first_frames, float *const *outs, int32_t n, hfx_stream_t stream) {
+  if (!plans || !rings || !ring_frames || !first_frames || !outs) return fail(HFX_E_INVALID, "null argument");
+  static_assert(HFX_GROUP_MAX <= hfx::HFX_MAX_WIN, "grouped launches carry at most HFX_MAX_WIN windows");
+  if (n < 1 || n > HFX_GROUP_MAX)
+    return fail(HFX_E_INVALID, "a group holds 1 to " + std::to_string(HFX_GROUP_MAX) + " windows");
+  hfx_plan *p0 = plans[0];
+  for (int w = 0; w < n; ++w) {
+    if (!plans[w] || !outs[w]) return fail(HFX_E_INVALID, "null plan or output in the group");
+    HFX_TRY(check_ring(plans[w], rings[w], ring_frames[w], first_frames[w]));
+    for (int v = 0; v < w; ++v)
+      if (plans[v] == plans[w]) return fail(HFX_E_INVALID, "a plan appears twice in the group (one workspace each)");
+    if (!group_compatible(p0, plans[w]))
+      return fail(HFX_E_INVALID, "group plans differ in geometry, window, band, weights or options");
+  }
+  hipStream_t st = S(stream);
+  if (n == 1) return window_moments(p0, rings[0], ring_frames[0], first_frames[0], outs[0], st);
+  if (!p0->fused_row || p0->filter2d || p0->ff ||
+      !hfx::dft_front_pipelined(p0->W, p0->nbc, p0->d.window, p0->n_planes, p0->d.precision) ||
+      !hfx::wave_pass_supported(p0->OH) || !hfx::wave_pass_supported(p0->OW))
+    return fail(HFX_E_UNSUPPORTED, "grouped windows need the pipelined FP32 front (width 512, 768 or 1024), "
+                                   "wave-FFT sizes and no Filter2D or Flatfield");
+  // front: one launch over the windows of all plans (each into its own plan's workspace)
+  hfx::DftLaunch L = dft_launch(p0, rings[0], ring_frames[0], first_frames[0], p0->planes_blk);
+  L.nwin = n;
+  const bool fresnel = p0->d.prop == HFX_PROP_FRESNEL;
+  for (int w = 0; w < n; ++w) {
+    L.firsts[w] = first_frames[w];
+    L.rings[w] = rings[w];
+    L.ring_frames_w[w] = ring_frames[w];
+    L.planes_w[w] = plans[w]->planes_blk;
+    L.lx_w[w] = fresnel ? plans[w]->lx : nullptr;
+    L.ly_w[w] = fresnel ? plans[w]->ly : nullptr;
+  }
+  if (hfx::launch_time_dft_row(L, p0->OH, p0->blk_c, p0->n_planes, p0->tw_w, st) != 0)
+    return fail(HFX_E_HIP, "grouped front launch failed");
+  // back: one launch per pass, blockIdx.z picking the window
+  hfx::MomentArgs mom{};
+  mom.f = p0->wf;
+  mom.f2 = p0->wf2;
+  mom.split = p0->d.split_bin > 0 ? p0->d.split_bin - p0->d.bin_begin : 0;
+  mom.power2 = (p0->d.flags & HFX_FLAG_POWER2) ? 1 : 0;
+  mom.shift = (p0->d.flags & HFX_FLAG_FFTSHIFT) ? 1 : 0;
+  auto set_windows = [&](hfx::PassArgs &a, bool mid) {
+    a.nwin = n;
+    for (int w = 0; w < n; ++w) {
+      a.win[w].in = plans[w]->planes_blk;
+      a.win[w].out = mid ? plans[w]->planes_blk : nullptr;
+      a.win[w].mid_tf = mid ? plans[w]->tf : nullptr;
+      for (int i = 0; i < 5; ++i) a.win[w].mom_out[i] = i < p0->n_out ? outs[w] + i * p0->onpix : nullptr;
+    }
+    // the single-window fields stay consistent with window 0 (kinds are chosen from them)
+    a.in = a.win[0].in;
+    a.out = a.win[0].out;
+    a.mid_tf = a.win[0].mid_tf;
+    for (int i = 0; i < 5; ++i) a.mom.out[i] = a.win[0].mom_out[i];
+  };
+  hfx::PassArgs c = base_pass(p0->OH, p0->OW, p0->n_planes, p0->blk_c);
+  c.in_mode = hfx::IN_BLK;
+  c.tw = p0->tw_h;
+  if (fresnel) {
+    c.out_mode = hfx::OUT_MOMENTS;
+    c.mom = mom;
+    set_windows(c, false);
+    if (hfx::launch_col_pass(c, st) != 0) return fail(HFX_E_HIP, "grouped column pass launch failed");
+    return HFX_OK;
+  }
+  c.out_mode = hfx::OUT_BLK;
+  set_windows(c, true);
+  if (hfx::launch_col_pass(c, st) != 0) return fail(HFX_E_HIP, "grouped column pass launch failed");
+  hfx::PassArgs r2 = base_pass(p0->OH, p0->OW, p0->n_planes, p0->blk_c);
+  r2.in_mode = hfx::IN_BLK;
+  r2.inv = 1;
+  r2.out_mode = hfx::OUT_MOMENTS;
+  r2.mom = mom;
+  r2.tw = p0->tw_w;
+  set_windows(r2, false);
+  if (hfx::launch_row_pass(r2, st) != 0) return fail(HFX_E_HIP, "grouped inverse row pass launch failed");
+  return HFX_OK;
 }
 
 hfx_status hfx_plan_run_stage(hfx_plan *p, int32_t stage, const uint16_t *ring, int64_t ring_frames,

@@ -37,10 +37,14 @@ struct DftArgs {
   // k_dft_row only
   int H, blk_c, n_planes;
   const float2 *tw;        // W-point FFT twiddles
-  // k_front only: windows of one launch (first frames, planes at planes + w * planes_stride)
+  // k_front only: the windows of one launch (one plan's consecutive windows, or one window
+  // each of a group of plans of one shape): ring, ring length, first frame, planes, lens
   int nwin;
   long long firsts[HFX_MAX_WIN];
-  long long planes_stride;
+  const uint16_t *rings[HFX_MAX_WIN];
+  long long ring_frames_w[HFX_MAX_WIN];
+  float2 *planes_w[HFX_MAX_WIN];
+  const float2 *lx_w[HFX_MAX_WIN], *ly_w[HFX_MAX_WIN];
 };
 
 // true for a computed bin that is the DC bin: bin 0, or bin T itself, the extra bin the
@@ -350,7 +354,7 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
   __syncthreads();
 
   const long long plane_elems = (long long)a.H * W;
-  const bool lens = a.lx != nullptr;
+  const bool lens = a.lx_w[0] != nullptr;
   const int G = gridDim.x;
   const int y0 = xcd_tile(blockIdx.x, G);
   const int nrw = y0 < a.H ? (a.H - 1 - y0) / G + 1 : 0;  // rows of one window
@@ -361,9 +365,13 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
   std::conditional_t<FOLD, DftStreamFold<TPW, T, 4>, DftStream<NBT, TPW, T, front_ring(T)>> stream;
   auto px0 = [&](int y) { return (long long)y * W + rw * 64 * TPW; };
   auto row_y = [&](int i) { return y0 + (nrw ? i % nrw : 0) * G; };
+  auto win_of = [&](int i) { return nrw ? i / nrw : 0; };
   auto ring_of = [&](int i) {
+    const int w = win_of(i);
     DftRing r = a.r;
-    r.first = a.firsts[nrw ? i / nrw : 0];
+    r.ring = a.rings[w];
+    r.ring_frames = a.ring_frames_w[w];
+    r.first = a.firsts[w];
     return r;
   };
 
@@ -377,8 +385,10 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
       stream.row(ring_of(i), ring_of(j), reinterpret_cast<const float *>(wmat), px0(row_y(i)), px0(row_y(j)),
                  has_next, lane, acc);
   };
-  // producers: MFMA registers -> padded LDS rows (lens applied for Fresnel)
-  auto scatter = [&](int y, int lane) {
+  // producers: MFMA registers of row i -> padded LDS rows (its window's lens for Fresnel)
+  auto scatter = [&](int i, int lane) {
+    const int y = row_y(i);
+    const float2 *lxw = a.lx_w[win_of(i)], *lyw = a.ly_w[win_of(i)];
     // band bin `bin` of 4 pixels from x0 into the LDS plane rows (and its conjugate plane on
     // the two-sided path), lens applied
     auto put = [&](int x0, int bin, const float2 (&v)[4], const float2 (&L)[4]) {
@@ -408,9 +418,9 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
       // lens factors; 1 without a lens (the product with (1, 0) is exact, and cheaper than a select)
       float2 L[4] = {make_float2(1.f, 0.f), make_float2(1.f, 0.f), make_float2(1.f, 0.f), make_float2(1.f, 0.f)};
       if (lens) {
-        const float2 ly = a.ly[y];
+        const float2 ly = lyw[y];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) L[t] = cmul(a.lx[x0 + t], ly);
+        for (int t = 0; t < 4; ++t) L[t] = cmul(lxw[x0 + t], ly);
       }
       if constexpr (FOLD) {
 #pragma unroll
@@ -445,7 +455,7 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
   // consumers: one wave per plane row: forward row FFT, then the column-blocked store
   auto transform = [&](int i, int lane) {
     const int y = row_y(i);
-    float2 *planes = a.planes + (nrw ? i / nrw : 0) * a.planes_stride;
+    float2 *planes = a.planes_w[win_of(i)];
     const long long row_off = blk_lane_offset(lane, a.H, a.blk_c) + (long long)y * a.blk_c;
     for (int q = rw; q < a.n_planes; q += front_cw<W>()) {
       float2 *ws = scr + q * F::SCR;
@@ -467,7 +477,7 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
     if (nr > 0) {
       stream.start(ring_of(0), px0(y0), lane0);
       dft_row(0, lane0);
-      scatter(y0, lane0);
+      scatter(0, lane0);
     }
     __syncthreads();
     for (int i = 0; i < nr; ++i) {
@@ -475,7 +485,7 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
       asm volatile("v_mov_b32 %0, %1" : "=v"(lane) : "v"(tid & 63));
       if (i + 1 < nr) dft_row(i + 1, lane);
       __syncthreads();  // consumers are done with row i's planes
-      if (i + 1 < nr) scatter(row_y(i + 1), lane);
+      if (i + 1 < nr) scatter(i + 1, lane);
       __syncthreads();  // row i + 1's planes are in LDS
     }
   } else {
@@ -514,9 +524,15 @@ static DftArgs make_args(const DftLaunch &L) {
   }
   a.wm_lo = L.wm_f16lo;
   a.nwin = L.nwin > 0 ? L.nwin : 1;
-  for (int w = 0; w < HFX_MAX_WIN; ++w) a.firsts[w] = w < L.nwin ? L.firsts[w] : L.first;
-  a.firsts[0] = L.first;
-  a.planes_stride = L.planes_stride;
+  for (int w = 0; w < HFX_MAX_WIN; ++w) {
+    const bool own = w < a.nwin && L.rings[w] != nullptr;  // a window of another plan of the group
+    a.firsts[w] = w == 0 ? L.first : w < a.nwin ? L.firsts[w] : L.first;
+    a.rings[w] = own ? L.rings[w] : L.ring;
+    a.ring_frames_w[w] = own ? L.ring_frames_w[w] : L.ring_frames;
+    a.planes_w[w] = own ? L.planes_w[w] : L.planes + (w < a.nwin ? w : 0) * L.planes_stride;
+    a.lx_w[w] = own ? L.lx_w[w] : L.lx;
+    a.ly_w[w] = own ? L.ly_w[w] : L.ly;
+  }
   return a;
 }
 

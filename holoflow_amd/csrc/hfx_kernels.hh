@@ -32,7 +32,7 @@ int launch_flatfield_spatial(const float *in, float *out, float *tmp, long long 
                              const float *kx, int rx, const float *ky, int ry, hipStream_t st);
 
 // ---- time DFT (dft.hip)
-constexpr int HFX_MAX_WIN = 4;  // windows per pipelined front launch
+constexpr int HFX_MAX_WIN = 4;  // windows per pipelined front launch (one plan, or a group of plans)
 
 struct DftLaunch {
   const uint16_t *ring;
@@ -42,6 +42,12 @@ struct DftLaunch {
   int nwin;
   long long firsts[HFX_MAX_WIN];
   long long planes_stride;
+  // grouped launch (windows of several plans of one shape): per-window ring, ring length,
+  // planes and Fresnel lens; null rings[w] means ring / ring_frames / planes + w * stride / lx, ly
+  const uint16_t *rings[HFX_MAX_WIN];
+  long long ring_frames_w[HFX_MAX_WIN];
+  float2 *planes_w[HFX_MAX_WIN];
+  const float2 *lx_w[HFX_MAX_WIN], *ly_w[HFX_MAX_WIN];
   int T;
   long long npix;
   int W;
@@ -73,6 +79,15 @@ struct MomentArgs {
   int shift;           // fftshift the stored images on both axes
 };
 
+// One window of a grouped pass launch (windows of several plans of one shape, blockIdx.z):
+// the pointers that differ between the windows; everything else is shared.
+struct PassWin {
+  const void *in;
+  void *out;
+  const float2 *mid_tf;
+  float *mom_out[5];
+};
+
 struct PassArgs {
   int H, W;                 // transform grid (output plane size)
   int planes;               // number of planes
@@ -93,6 +108,8 @@ struct PassArgs {
   MomentArgs mom;
   const float2 *tw;         // FFT twiddles for the transform length
   const float2 *tw2;        // second twiddle table unused by single-axis passes
+  int nwin;                 // > 1: grouped launch, window blockIdx.z takes win[blockIdx.z]
+  PassWin win[HFX_MAX_WIN];
 };
 
 int launch_row_pass(const PassArgs &a, hipStream_t st);

@@ -91,6 +91,7 @@ struct Stream {
   hfx_stream_desc d{};
   Geometry g{};
   int lane = 0;
+  bool groupable = false;  // its windows can join grouped launches (hfx_moments_group)
   hfx_plan *plan = nullptr;
   uint16_t *ring = nullptr, *pool = nullptr;
   float *outs = nullptr;
@@ -133,7 +134,8 @@ struct hfx_server {
   std::vector<int> lane_load;
   std::vector<std::deque<std::pair<int32_t, int64_t>>> lane_windows;  // EDF: (stream, window) per lane
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
-  int64_t bytes_held = 0, windows_launched = 0, windows_retired = 0;
+  int64_t bytes_held = 0, windows_launched = 0, windows_retired = 0, group_launches = 0, grouped_windows = 0;
+  int next_lane = 0;  // HFX_SCHED_GROUP: lanes taken in turn
   // GPU timestamps on the server clock: `epoch` was recorded on an idle stream and had
   // completed by host time epoch_s, so event e completed at or before
   // epoch_s + elapsed(epoch, e), late by at most epoch_err_s (the recording's round trip)
@@ -176,6 +178,9 @@ hfx_status build_stream(hfx_server *s, Stream &st) {
   hfx_plan_desc pd = st.d.plan;
   pd.batch_lanes = 1;  // windows run one at a time through hfx_moments: no second lane
   SRV_TRY(hfx_plan_create(&pd, cs, &st.plan));
+  hfx_plan_info info{};
+  SRV_TRY(hfx_plan_get_info(st.plan, &info));
+  st.groupable = info.groupable != 0;
   SRV_HIP(hipMalloc((void **)&st.ring, (size_t)g.ring_bytes));
   SRV_HIP(hipMalloc((void **)&st.pool, (size_t)g.pool_bytes));
   SRV_HIP(hipMalloc((void **)&st.outs, (size_t)g.out_bytes));
@@ -229,6 +234,81 @@ hfx_status launch_window(hfx_server *s, Stream &st, int lane_idx) {
   return HFX_OK;
 }
 
+// Windows of these two streams can share a grouped launch: the same plan shape (their z,
+// wavelength, pitch, cameras and rings may differ); hfx_moments_group checks the same.
+bool same_group(const Stream &a, const Stream &b) {
+  const hfx_plan_desc &x = a.d.plan, &y = b.d.plan;
+  return a.groupable && b.groupable && x.height == y.height && x.width == y.width && x.window == y.window &&
+         x.bin_begin == y.bin_begin && x.bin_end == y.bin_end && x.prop == y.prop && x.precision == y.precision &&
+         x.flags == y.flags && x.pad_height == y.pad_height && x.pad_width == y.pad_width &&
+         x.split_bin == y.split_bin && x.weight_fs_hz == y.weight_fs_hz && x.filter_enabled == y.filter_enabled &&
+         x.flatfield_enabled == y.flatfield_enabled;
+}
+
+// The next window of each member stream, in one launch on `lane`: each waits for its frames
+// and for its stream's previous window (which may have run on another lane: one plan
+// workspace per stream), a single window replays its captured graph, several run as one
+// hfx_moments_group launch.
+hfx_status launch_group(hfx_server *s, const std::vector<Stream *> &members, int lane_idx) {
+  hipStream_t lane = s->lanes[lane_idx];
+  const int n = (int)members.size();
+  std::vector<hfx_plan *> plans(n);
+  std::vector<const uint16_t *> rings(n);
+  std::vector<int64_t> rframes(n), firsts(n);
+  std::vector<float *> outs(n);
+  for (int i = 0; i < n; ++i) {
+    Stream &st = *members[i];
+    const int64_t w = st.launched, p = w % st.g.n_pos;
+    SRV_HIP(hipStreamWaitEvent(lane, st.ev_ready[p], 0));
+    if (w > 0) SRV_HIP(hipStreamWaitEvent(lane, st.ev_done[(w - 1) % st.g.n_pos], 0));
+    SRV_HIP(hipEventRecord(st.ev_launch[p], lane));
+    plans[i] = st.plan;
+    rings[i] = st.ring;
+    rframes[i] = st.g.R;
+    firsts[i] = p * st.g.S;
+    outs[i] = st.outs + p * st.g.out_elems;
+  }
+  if (n == 1 && s->d.use_graphs) {
+    SRV_HIP(hipGraphLaunch(members[0]->graphs[members[0]->launched % members[0]->g.n_pos], lane));
+  } else {
+    SRV_TRY(hfx_moments_group(plans.data(), rings.data(), rframes.data(), firsts.data(), outs.data(), n, lane));
+  }
+  for (int i = 0; i < n; ++i) {
+    Stream &st = *members[i];
+    SRV_HIP(hipEventRecord(st.ev_done[st.launched % st.g.n_pos], lane));
+    ++st.launched;
+    ++s->windows_launched;
+  }
+  if (n > 1) {
+    ++s->group_launches;
+    s->grouped_windows += n;
+  }
+  return HFX_OK;
+}
+
+// HFX_SCHED_GROUP: repeatedly take the ready window with the oldest due time and launch it
+// together with the oldest ready windows of up to HFX_GROUP_MAX - 1 other streams of the same
+// plan shape, on the next lane.
+hfx_status schedule_group(hfx_server *s, int32_t &n) {
+  for (;;) {
+    std::vector<std::pair<double, Stream *>> ready;
+    for (auto &up : s->slots) {
+      if (!up) continue;
+      Stream &st = *up;
+      if (st.launched < st.ready_windows()) ready.emplace_back(st.due_t[st.launched % st.g.n_pos], &st);
+    }
+    if (ready.empty()) return HFX_OK;
+    std::stable_sort(ready.begin(), ready.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+    std::vector<Stream *> members{ready[0].second};
+    for (size_t i = 1; i < ready.size() && (int)members.size() < HFX_GROUP_MAX; ++i)
+      if (same_group(*members[0], *ready[i].second)) members.push_back(ready[i].second);
+    const int lane = s->next_lane;
+    s->next_lane = (s->next_lane + 1) % (int)s->lanes.size();
+    SRV_TRY(launch_group(s, members, lane));
+    n += (int32_t)members.size();
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -267,7 +347,8 @@ hfx_status hfx_server_create(const hfx_server_desc *d, hfx_server **out) {
   if (d->max_streams < 1 || d->max_streams > 1024) return fail(HFX_E_INVALID, "max_streams must be in [1, 1024]");
   if (d->n_lanes < 1 || d->n_lanes > 16) return fail(HFX_E_INVALID, "n_lanes must be in [1, 16]");
   if (d->hbm_budget_bytes < 0 || d->hbm_reserve_bytes < 0) return fail(HFX_E_INVALID, "negative memory size");
-  if (d->policy != HFX_SCHED_FIFO && d->policy != HFX_SCHED_EDF) return fail(HFX_E_INVALID, "unknown policy");
+  if (d->policy != HFX_SCHED_FIFO && d->policy != HFX_SCHED_EDF && d->policy != HFX_SCHED_GROUP)
+    return fail(HFX_E_INVALID, "unknown policy");
   auto s = std::make_unique<hfx_server>();
   s->d = *d;
   SRV_HIP(hipGetDevice(&s->device));
@@ -436,6 +517,8 @@ hfx_status hfx_server_schedule(hfx_server *s, int32_t *launched) {
   int32_t n = 0;
   if (s->d.policy == HFX_SCHED_EDF) {
     SRV_TRY(schedule_edf(s, n));
+  } else if (s->d.policy == HFX_SCHED_GROUP) {
+    SRV_TRY(schedule_group(s, n));
   } else {
     // every ready window, oldest camera due time first across the streams (a stream's own
     // windows keep their order: their due times do not decrease)
@@ -522,6 +605,8 @@ hfx_status hfx_server_info_get(hfx_server *s, hfx_server_info *info) {
   info->windows_launched = s->windows_launched;
   info->windows_retired = s->windows_retired;
   info->clock_error_us = s->epoch_err_s * 1e6;
+  info->group_launches = s->group_launches;
+  info->grouped_windows = s->grouped_windows;
   return HFX_OK;
 }
 
@@ -648,6 +733,8 @@ hfx_status hfx_serve_workload(const hfx_workload_desc *d, hfx_workload_result *r
   }
   r->wall_s = last_retire - (t0 + d->arrival_s[0]);
   r->clock_error_us = srv->epoch_err_s * 1e6;
+  r->group_launches = srv->group_launches;
+  r->grouped_windows = srv->grouped_windows;
   return HFX_OK;
 }
 

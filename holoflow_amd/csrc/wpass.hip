@@ -53,7 +53,20 @@ template <int E, int NACC> struct Moments {
   }
 };
 
-__device__ __forceinline__ void store_moment(const PassArgs &a, int nacc, int y, int x, const float (&val)[5]) {
+// The pointers of this block's window: win[blockIdx.z] of a grouped launch, else the pass's own.
+__device__ __forceinline__ PassWin pass_window(const PassArgs &a) {
+  if (a.nwin > 1) return a.win[blockIdx.z];
+  PassWin w;
+  w.in = a.in;
+  w.out = a.out;
+  w.mid_tf = a.mid_tf;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) w.mom_out[k] = a.mom.out[k];
+  return w;
+}
+
+__device__ __forceinline__ void store_moment(const PassArgs &a, float *const (&mo)[5], int nacc, int y, int x,
+                                             const float (&val)[5]) {
   int ys = y, xs = x;  // 0 <= y < H, 0 <= x < W
   if (a.mom.shift) {   // (y + H - H/2) mod H, without a division
     ys = y + (a.H - a.H / 2);
@@ -63,12 +76,12 @@ __device__ __forceinline__ void store_moment(const PassArgs &a, int nacc, int y,
   }
   const long long o = (long long)ys * a.W + xs;
   const float n_all = (float)a.planes;
-  a.mom.out[0][o] = val[0] / n_all;
-  a.mom.out[1][o] = val[1] / n_all;
-  a.mom.out[2][o] = val[2] / n_all;
+  mo[0][o] = val[0] / n_all;
+  mo[1][o] = val[1] / n_all;
+  mo[2][o] = val[2] / n_all;
   if (nacc == 5) {
-    if (a.mom.out[3]) a.mom.out[3][o] = val[3] / (float)a.mom.split;
-    if (a.mom.out[4]) a.mom.out[4][o] = val[4] / (float)(a.planes - a.mom.split);
+    if (mo[3]) mo[3][o] = val[3] / (float)a.mom.split;
+    if (mo[4]) mo[4][o] = val[4] / (float)(a.planes - a.mom.split);
   }
 }
 
@@ -95,6 +108,7 @@ __global__ __launch_bounds__(64 * wrow_waves(N, KIND)) void k_wrow(PassArgs a) {
 
   const int y = xcd_tile(blockIdx.x, gridDim.x) * RW + wave;
   if (y >= a.H) return;
+  const PassWin pw = pass_window(a);
   const int C = a.blk_c;
   const long long plane_elems = (long long)a.H * a.W;
   const long long boff = blk_lane_offset(lane, a.H, C) + (long long)y * C;  // blocked (y, lane)
@@ -110,14 +124,14 @@ __global__ __launch_bounds__(64 * wrow_waves(N, KIND)) void k_wrow(PassArgs a) {
         if (yi >= 0 && yi < a.in_h && xi >= 0 && xi < a.in_w) {
           const long long o = p * (long long)a.in_h * a.in_w + (long long)yi * a.in_w + xi;
           if (a.in_mode == IN_NAT_C)
-            t = reinterpret_cast<const float2 *>(a.in)[o];
+            t = reinterpret_cast<const float2 *>(pw.in)[o];
           else
-            t = make_float2(reinterpret_cast<const float *>(a.in)[o], 0.f);
+            t = make_float2(reinterpret_cast<const float *>(pw.in)[o], 0.f);
           if (a.pre_lx) t = cmul(t, cmul(a.pre_lx[xi], a.pre_ly[yi]));
         }
         v[m] = t;
       } else {
-        v[m] = reinterpret_cast<const float2 *>(a.in)[p * plane_elems + boff + (long long)m * 64 * a.H];
+        v[m] = reinterpret_cast<const float2 *>(pw.in)[p * plane_elems + boff + (long long)m * 64 * a.H];
       }
     }
   };
@@ -153,7 +167,7 @@ __global__ __launch_bounds__(64 * wrow_waves(N, KIND)) void k_wrow(PassArgs a) {
         val[3] = acc.s[3][m];
         val[4] = acc.s[4][m];
       }
-      store_moment(a, NACC, y, lane + 64 * m, val);
+      store_moment(a, pw.mom_out, NACC, y, lane + 64 * m, val);
     }
   } else {
     const long long p = blockIdx.y;
@@ -165,16 +179,16 @@ __global__ __launch_bounds__(64 * wrow_waves(N, KIND)) void k_wrow(PassArgs a) {
       const int x = lane + 64 * m;
       float2 t = v[m];
       if (KIND == NAT_TO_BLK) {
-        reinterpret_cast<float2 *>(a.out)[p * plane_elems + boff + (long long)m * 64 * a.H] = t;
+        reinterpret_cast<float2 *>(pw.out)[p * plane_elems + boff + (long long)m * 64 * a.H] = t;
       } else {
         const long long o = p * plane_elems + (long long)y * a.W + x;
         if (a.out_mode == OUT_NAT_MAG) {
-          reinterpret_cast<float *>(a.out)[o] = sqrtf(t.x * t.x + t.y * t.y);
+          reinterpret_cast<float *>(pw.out)[o] = sqrtf(t.x * t.x + t.y * t.y);
         } else if (a.out_mode == OUT_NAT_RE) {
-          reinterpret_cast<float *>(a.out)[o] = t.x;
+          reinterpret_cast<float *>(pw.out)[o] = t.x;
         } else {
           if (a.post_lx) t = cmul(t, cmul(a.post_lx[x], a.post_ly[y]));
-          reinterpret_cast<float2 *>(a.out)[o] = t;
+          reinterpret_cast<float2 *>(pw.out)[o] = t;
         }
       }
     }
@@ -229,12 +243,13 @@ __global__ __launch_bounds__(wcol_threads(N, KIND)) void k_wcol(PassArgs a) {
   // element (lane + 64 m) of the column: tile + lane * C + c + m * col_step, the per-m step a
   // wave-uniform scalar (no per-element 64-bit products kept live across the transforms)
   const long long col_step = 64LL * C;
+  const PassWin pw = pass_window(a);
   float2 *ws = scr + wave * F::SCR;
 
   auto load = [&](float2 (&v)[E], long long p) {
 #pragma unroll
     for (int m = 0; m < E; ++m)
-      v[m] = reinterpret_cast<const float2 *>(a.in)[p * plane_elems + tile + lane * C + c + m * col_step];
+      v[m] = reinterpret_cast<const float2 *>(pw.in)[p * plane_elems + tile + lane * C + c + m * col_step];
   };
 
   if constexpr (is_mom(KIND)) {
@@ -289,7 +304,7 @@ __global__ __launch_bounds__(wcol_threads(N, KIND)) void k_wcol(PassArgs a) {
           val[3] = acc.s[3][m];
           val[4] = acc.s[4][m];
         }
-        store_moment(a, NACC, lane + 64 * m, x, val);
+        store_moment(a, pw.mom_out, NACC, lane + 64 * m, x, val);
       }
     }
   } else {
@@ -300,11 +315,11 @@ __global__ __launch_bounds__(wcol_threads(N, KIND)) void k_wcol(PassArgs a) {
         F::template run<false>(v, ws, twl, lane);
         asm volatile("" ::: "memory");  // keep the table loads after the forward transform
 #pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], a.mid_tf[tile + lane * C + c + m * col_step]);
+        for (int m = 0; m < E; ++m) v[m] = cmul(v[m], pw.mid_tf[tile + lane * C + c + m * col_step]);
         F::template run<true>(v, ws, twl, lane);
 #pragma unroll
         for (int m = 0; m < E; ++m)
-          reinterpret_cast<float2 *>(a.out)[p * plane_elems + tile + lane * C + c + m * col_step] = v[m];
+          reinterpret_cast<float2 *>(pw.out)[p * plane_elems + tile + lane * C + c + m * col_step] = v[m];
       } else {
         F::template run<false>(v, ws, twl, lane);
 #pragma unroll
@@ -313,12 +328,12 @@ __global__ __launch_bounds__(wcol_threads(N, KIND)) void k_wcol(PassArgs a) {
           float2 t = v[m];
           const long long o = p * plane_elems + (long long)y * a.W + x;
           if (a.out_mode == OUT_NAT_MAG) {
-            reinterpret_cast<float *>(a.out)[o] = sqrtf(t.x * t.x + t.y * t.y);
+            reinterpret_cast<float *>(pw.out)[o] = sqrtf(t.x * t.x + t.y * t.y);
           } else if (a.out_mode == OUT_NAT_RE) {
-            reinterpret_cast<float *>(a.out)[o] = t.x;
+            reinterpret_cast<float *>(pw.out)[o] = t.x;
           } else {
             if (a.post_lx) t = cmul(t, cmul(a.post_lx[x], a.post_ly[y]));
-            reinterpret_cast<float2 *>(a.out)[o] = t;
+            reinterpret_cast<float2 *>(pw.out)[o] = t;
           }
         }
       }
@@ -353,7 +368,8 @@ template <int N, int KIND> static int launch_wrow_n(const PassArgs &a, hipStream
   constexpr int RW = wrow_waves(N, KIND);
   const unsigned gx = (unsigned)((a.H + RW - 1) / RW);
   const unsigned gy = is_mom(KIND) ? 1u : (unsigned)a.planes;
-  hipLaunchKernelGGL((k_wrow<N, KIND>), dim3(gx, gy), dim3(64 * RW), 0, st, a);
+  const unsigned gz = a.nwin > 1 ? (unsigned)a.nwin : 1u;
+  hipLaunchKernelGGL((k_wrow<N, KIND>), dim3(gx, gy, gz), dim3(64 * RW), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -363,7 +379,8 @@ template <int N, int KIND> static int launch_wcol_n(const PassArgs &a, hipStream
   const unsigned gx = (unsigned)((a.W / a.blk_c + TL - 1) / TL);
   const unsigned gy = is_mom(KIND) ? 1u : (unsigned)((a.planes + wcol_ppb(N, KIND) - 1) / wcol_ppb(N, KIND));
   const unsigned nt = wcol_threads(N, KIND);
-  hipLaunchKernelGGL((k_wcol<N, KIND>), dim3(gx, gy), dim3(nt), 0, st, a);
+  const unsigned gz = a.nwin > 1 ? (unsigned)a.nwin : 1u;
+  hipLaunchKernelGGL((k_wcol<N, KIND>), dim3(gx, gy, gz), dim3(nt), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

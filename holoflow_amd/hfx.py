@@ -176,6 +176,26 @@ class MomentPlan:
                                       stride, _ptr(out), _stream(stream)))
         return out
 
+    @staticmethod
+    def moments_group(plans, rings, first_frames, outs=None, stream=None):
+        """One window of each plan in one front launch and one launch per back pass
+        (hfx_moments_group): plans of one shape whose z, wavelength or pitch may differ.
+        Returns the list of outputs ([n_out][OH][OW] each)."""
+        n = len(plans)
+        if not (len(rings) == len(first_frames) == n):
+            raise ValueError("plans, rings and first_frames must have the same length")
+        outs = [p.new_output() for p in plans] if outs is None else list(outs)
+        for p, r, o in zip(plans, rings, outs):
+            check_ring(r, p.cfg.height, p.cfg.width, p.cfg.window)
+            check_out(o, 0, p.out_shape)
+        arr = lambda T, xs: (T * n)(*xs)  # noqa: E731
+        check(lib().hfx_moments_group(arr(C.c_void_p, [p._h for p in plans]),
+                                      arr(C.c_void_p, [_ptr(r) for r in rings]),
+                                      arr(C.c_int64, [r.shape[0] for r in rings]),
+                                      arr(C.c_int64, [int(f) for f in first_frames]),
+                                      arr(C.c_void_p, [_ptr(o) for o in outs]), n, _stream(stream)))
+        return outs
+
     def run_stage(self, stage: int, ring: Optional[torch.Tensor], first_frame: int = 0,
                   out: Optional[torch.Tensor] = None, stream=None) -> None:
         """Stage 0 (time DFT + row FFT into the workspace) or stage 1 (rest -> out)."""

@@ -124,7 +124,7 @@ def server_desc(max_streams=16, n_lanes=4, budget_bytes=0, reserve_bytes=8 << 30
     d.max_streams, d.n_lanes = max_streams, n_lanes
     d.hbm_budget_bytes, d.hbm_reserve_bytes = budget_bytes, reserve_bytes
     d.use_graphs = int(use_graphs)
-    d.policy = {"fifo": _lib.SCHED_FIFO, "edf": _lib.SCHED_EDF}[policy]
+    d.policy = {"fifo": _lib.SCHED_FIFO, "edf": _lib.SCHED_EDF, "group": _lib.SCHED_GROUP}[policy]
     d.lane_depth = lane_depth
     return d
 
@@ -144,6 +144,8 @@ class WorkloadResult:
     capture: Optional[np.ndarray]
     windows_ingested_late: int = 0
     clock_error_us: float = 0.0
+    group_launches: int = 0      # launches that carried windows of several streams (policy "group")
+    grouped_windows: int = 0
 
 
 def run_workload(specs: Sequence[StreamSpec], arrival_s: Sequence[float], window_rate_hz: Sequence[float],
@@ -187,7 +189,7 @@ def run_workload(specs: Sequence[StreamSpec], arrival_s: Sequence[float], window
     check(lib().hfx_serve_workload(C.byref(d), C.byref(r)))
     return WorkloadResult(r.wall_s, r.windows_retired, r.max_concurrent, r.admissions_deferred,
                           r.ingest_deferred, lat, gpu, wait, adm, fin, cap, r.windows_ingested_late,
-                          r.clock_error_us)
+                          r.clock_error_us, r.group_launches, r.grouped_windows)
 
 
 def summarize(specs: Sequence[StreamSpec], res: WorkloadResult, periods_s=None):
@@ -219,6 +221,8 @@ def summarize(specs: Sequence[StreamSpec], res: WorkloadResult, periods_s=None):
         "ingest_deferred": int(res.ingest_deferred),
         "windows_ingested_late": int(res.windows_ingested_late),
         "clock_error_us": round(float(res.clock_error_us), 1),
+        "group_launches": int(res.group_launches),
+        "grouped_windows": int(res.grouped_windows),
     }
     return out
 

@@ -98,6 +98,7 @@ typedef struct hfx_plan_info {
   int32_t n_planes;              /* propagated planes (nb, or 2*nb for both sidebands) */
   int64_t workspace_bytes;       /* device bytes owned by the plan */
   int32_t batch_lanes;           /* streams hfx_moments_batch alternates over (1 or 2) */
+  int32_t groupable;             /* 1: the plan's windows can join hfx_moments_group launches */
 } hfx_plan_info;
 
 typedef struct hfx_plan hfx_plan;
@@ -124,6 +125,18 @@ hfx_status hfx_moments(hfx_plan *p, const uint16_t *ring, int64_t ring_frames,
 hfx_status hfx_moments_batch(hfx_plan *p, const uint16_t *ring, int64_t ring_frames,
                              int64_t first_frame, int32_t n_windows, int64_t out_stride,
                              float *out, hfx_stream_t stream);
+/* One window each of n plans (n <= HFX_GROUP_MAX, each plan at most once) in one front launch
+ * and one launch per back pass: windows of different acquisitions batched together.  The
+ * plans must agree on frame size, window, band bins, weights, propagation kind, precision
+ * and flags (their z, wavelength and pitch may differ) and run the pipelined FP32 front
+ * (frame width 512, 768 or 1024, up to 16 bins, T in {32, 64, 128}); HFX_E_INVALID names a
+ * mismatch, HFX_E_UNSUPPORTED a geometry without the grouped kernels.  Window w reads
+ * rings[w] (ring_frames[w] frames, from first_frames[w]) through plans[w]'s tables and
+ * workspace and writes outs[w]; the results equal hfx_moments on each plan alone.  No
+ * counterpart in the reference (one acquisition per process, manager.cc). */
+#define HFX_GROUP_MAX 4
+hfx_status hfx_moments_group(hfx_plan *const *plans, const uint16_t *const *rings, const int64_t *ring_frames,
+                             const int64_t *first_frames, float *const *outs, int32_t n, hfx_stream_t stream);
 /* Time-DFT stage of the plan alone (RFFT -> band slice, with the Fresnel lens applied
  * when the plan propagates with Fresnel): writes n_planes complex64 planes [n_planes][H][W]
  * to planes_out (device). */

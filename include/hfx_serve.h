@@ -57,9 +57,15 @@ typedef struct hfx_server_desc {
  *      the ready window with the earliest deadline (due time + the stream's window period
  *      stride / fs_hz) of a stream with no window on the GPU is launched there.  A stream has
  *      at most one window in flight (its plan workspace), so its windows keep their order.
+ * GROUP: the ready window with the oldest due time goes out together with the oldest ready
+ *       windows of up to HFX_GROUP_MAX - 1 other streams of the same plan shape (frame size,
+ *       window, band, weights, propagation kind; z, wavelength, pitch may differ) as one
+ *       hfx_moments_group launch, on the lanes in turn; each window waits for its stream's
+ *       previous one, so a stream's windows still run in order.  Single windows replay their
+ *       captured graphs.
  * FIFO is the default: under the mixed 70 %-load workload EDF kept the same share of windows
  * within their period but doubled p99 (bursty streams wait on their own single slot). */
-enum hfx_sched_policy { HFX_SCHED_FIFO = 0, HFX_SCHED_EDF = 1 };
+enum hfx_sched_policy { HFX_SCHED_FIFO = 0, HFX_SCHED_EDF = 1, HFX_SCHED_GROUP = 2 };
 
 typedef struct hfx_stream_desc {
   hfx_plan_desc plan;        /* the stream's moment plan (frame size, T, S, band, optics) */
@@ -84,6 +90,8 @@ typedef struct hfx_server_info {
   int64_t windows_launched;
   int64_t windows_retired;
   double clock_error_us;     /* bound on how late GPU times read on the server clock are */
+  int64_t group_launches;    /* HFX_SCHED_GROUP: launches that carried windows of several streams */
+  int64_t grouped_windows;   /* windows those launches carried */
 } hfx_server_info;
 
 void hfx_server_desc_init(hfx_server_desc *d);
@@ -155,6 +163,8 @@ typedef struct hfx_workload_result {
   float *wait_us;                  /* caller array n_streams * windows_per_stream (or NULL) */
   int64_t windows_ingested_late;   /* windows ingested more than one period after their due time */
   double clock_error_us;           /* the server's GPU-time bound (hfx_server_info.clock_error_us) */
+  int64_t group_launches;          /* as in hfx_server_info */
+  int64_t grouped_windows;
 } hfx_workload_result;
 
 hfx_status hfx_serve_workload(const hfx_workload_desc *d, hfx_workload_result *r);

@@ -162,3 +162,30 @@ def test_latency_counts_from_the_camera_due_time(gpu):
     srv.release(sid)
     srv.close()
 
+
+
+def _rig(n, H=512, W=512, T=32, prop="asm", fs=37e3):
+    """n acquisitions of one plan shape (frame size, window, band) with different z and
+    cameras, as a multi-camera rig delivers them."""
+    kb, ke = serving.band_bins(6e3, 15e3, fs, T)
+    return [serving.StreamSpec(H, W, T, T // 4, kb, ke, 0.2 + 0.07 * i, fs, seed=500 + i, stream_id=i, prop=prop)
+            for i in range(n)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prop,k", [("asm", 0), ("asm", 3), ("fresnel", 2)])
+def test_grouped_launches_match_each_stream_alone(gpu, prop, k):
+    """Policy "group": ready windows of streams of one plan shape go out as one
+    hfx_moments_group launch; each stream's moments equal those of the stream served alone
+    and of eager hfx_moments calls on its frames."""
+    specs = _rig(4, prop=prop)
+    wps = 10
+    mix = serving.run_workload(specs, [0.0] * 4, [3000.0] * 4, wps, capture_stream=k, arrivals="periodic",
+                               server=serving.server_desc(max_streams=4, n_lanes=2, policy="group"),
+                               time_limit_s=60)
+    assert mix.windows_retired == 4 * wps
+    assert mix.group_launches > 0 and mix.grouped_windows >= 2 * mix.group_launches
+    alone = serving.run_workload([specs[k]], [0.0], [3000.0], wps, capture_stream=0, arrivals="periodic",
+                                 server=serving.server_desc(max_streams=1, n_lanes=1), time_limit_s=60)
+    assert np.array_equal(alone.capture, mix.capture)
+    assert np.array_equal(alone.capture, _eager(specs[k], wps))
