@@ -201,11 +201,42 @@ def serving_leg(args, np, torch, slots=16, policy="fifo"):
         if summ["latency_over_window_period_p99"] <= 1.0 and (best is None or load > best):
             best = load
     res["highest_load_with_p99_within_period"] = best
+    res["rig"] = rig_leg(args, np)
     res["note"] = ("latency = camera due time of the window's last frame -> moments complete on the GPU "
                    "(GPU events read on the host clock, never early); period = stride / (camera rate "
                    "at the run's slowdown); offered_load_at_camera_rates = GPU time 16 streams of this "
                    "mix need per second at their own cameras' rates")
     return res
+
+
+def rig_leg(args, np, slots=16):
+    """A multi-camera rig: 16 acquisitions of one plan shape (512x512, T = 32, S = 8, 37 kHz,
+    band 6-15 kHz, angular spectrum) at different z, periodic cameras slowed to offer each
+    load in --serving-loads of one GPU (from the window cost alone), served with FIFO launches
+    and with policy "group" (ready windows of the same shape batched into one launch)."""
+    from holoflow_amd import serving
+
+    T, fs = 32, 37e3
+    kb, ke = serving.band_bins(6e3, 15e3, fs, T)
+    specs = [serving.StreamSpec(512, 512, T, T // 4, kb, ke, 0.2 + 0.025 * i, fs, seed=700 + i, stream_id=i)
+             for i in range(slots)]
+    cost = serving.window_cost_us(specs[0])
+    out = {"streams": slots, "shape": "512x512 T=32 S=8 ASM, z 0.2-0.575 m", "window_cost_us_alone": round(cost, 1),
+           "offered_load_at_camera_rates": round(slots * cost * 1e-6 * fs / (T // 4), 3), "loads": {}}
+    for load in args.serving_loads + [0.9, 1.1]:
+        rate = load / (slots * cost * 1e-6)
+        row = {}
+        for policy in ("fifo", "group"):
+            r = serving.run_workload(specs, [0.0] * slots, [rate] * slots, args.serving_windows,
+                                     server=serving.server_desc(max_streams=slots, n_lanes=4, policy=policy),
+                                     seed=5, time_limit_s=240.0, arrivals="periodic")
+            sm = serving.summarize(specs, r, periods_s=[1.0 / rate] * slots)
+            row[policy] = {k: sm[k] for k in ("windows_per_s", "latency_us_p50", "latency_us_p99",
+                                              "latency_over_window_period_p99", "frac_windows_within_period",
+                                              "group_launches", "grouped_windows")}
+        out["loads"][str(load)] = row
+        log(f"rig load {load}: {row}")
+    return out
 
 
 def stages_leg(args, np, torch, ring, n_windows, cfg, value):
