@@ -223,7 +223,7 @@ def rig_leg(args, np, slots=16):
     cost = serving.window_cost_us(specs[0])
     out = {"streams": slots, "shape": "512x512 T=32 S=8 ASM, z 0.2-0.575 m", "window_cost_us_alone": round(cost, 1),
            "offered_load_at_camera_rates": round(slots * cost * 1e-6 * fs / (T // 4), 3), "loads": {}}
-    for load in args.serving_loads + [0.9, 1.1]:
+    for load in (0.5, 0.9, 1.2):
         rate = load / (slots * cost * 1e-6)
         row = {}
         for policy in ("fifo", "group"):
@@ -362,7 +362,7 @@ def main():
     ap.add_argument("--cfg4-frames", type=int, default=8192)
     ap.add_argument("--serving-streams", type=int, default=24)
     ap.add_argument("--serving-windows", type=int, default=128)
-    ap.add_argument("--serving-loads", type=float, nargs="+", default=[0.3, 0.5, 0.7])
+    ap.add_argument("--serving-loads", type=float, nargs="+", default=[0.5, 0.7, 0.9])
     ap.add_argument("--kernel-loop", type=int, default=200,
                     help="launches of the dominant kernel timed with HIP events")
     ap.add_argument("--batch-lanes", type=int, default=0,

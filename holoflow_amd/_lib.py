@@ -224,6 +224,7 @@ SIGNATURES.update({
     "hfx_server_destroy": (C.c_int, [C.c_void_p]),
     "hfx_server_admit": (C.c_int, [C.c_void_p, C.POINTER(StreamDesc), C.POINTER(C.c_int32)]),
     "hfx_server_clock": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "hfx_server_stream_state": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
     "hfx_server_ingest": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64]),
     "hfx_server_ingest_due": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_double]),
     "hfx_server_schedule": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),

@@ -1,12 +1,18 @@
 // Serving loop for concurrent acquisition streams on one GPU (include/hfx_serve.h).
 //
 // Per admitted stream the device holds: a frame ring of R = T + q*S frames (q = queued
-// windows), a source pool of P frames the synthetic camera plays from (generated once at
-// admission by hfx_synth_frames, so ingest is a device-to-device copy, as a frame grabber's
-// DMA would be), the moment plan (tables + workspace) and one output slot per window
+// windows), a source pool of P = T frames the synthetic camera plays cyclically (generated
+// once at set-up by hfx_synth_frames, so ingest is a device-to-device copy, as a frame
+// grabber's DMA would be; the pool is kept to one window because generating it is GPU work
+// that a real camera does not add, and a 4T pool measurably delayed the windows of the
+// streams already being served), the moment plan (tables + workspace) and one output slot per window
 // position (n_pos = R / S positions).  Window w reads ring frames [w*S, w*S + T) mod R,
 // which for R = n_pos*S is position p = w mod n_pos starting at p*S: every position is
 // captured once into a hipGraph (front + back kernels of hfx_moments) and replayed.
+//
+// Set-up and tear-down of a stream (device allocations, plan tables, the camera's source
+// pool, graph capture; waiting for its last work, frees) run on the server's worker thread,
+// so the serving loop never blocks on them: a stream is usable once its state is ready.
 //
 // Ordering: the camera copies on one ingest stream and records `ready[p]` after the last
 // frame of window w; the window's lane waits on it, records `launch[p]`, replays the graph
@@ -16,13 +22,17 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "hfx.h"
@@ -70,7 +80,7 @@ hfx_status geometry(const hfx_stream_desc &d, Geometry &g) {
   g.S = p.stride;
   g.n_pos = g.T / g.S + d.queue_windows;
   g.R = g.n_pos * g.S;
-  g.P = 4 * g.T;
+  g.P = g.T;  // the synthetic camera plays a pool of T frames cyclically
   g.npix = (int64_t)p.height * p.width;
   g.oh = p.pad_height > 0 ? p.pad_height : p.height;
   g.ow = p.pad_width > 0 ? p.pad_width : p.width;
@@ -100,6 +110,10 @@ struct Stream {
   std::vector<hipEvent_t> ev_ready, ev_launch, ev_done;
   std::vector<double> due_t;  // server-clock time (s) the camera delivered each position's last frame
   int64_t frames_in = 0, launched = 0, retired = 0;
+  // set up by the worker thread: 0 building, 1 ready, 2 failed (error holds the reason); the
+  // worker writes every other member before it stores 1 or 2
+  std::atomic<int> state{0};
+  std::string error;
 
   int64_t ready_windows() const { return frames_in >= g.T ? (frames_in - g.T) / g.S + 1 : 0; }
 
@@ -136,6 +150,17 @@ struct hfx_server {
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   int64_t bytes_held = 0, windows_launched = 0, windows_retired = 0, group_launches = 0, grouped_windows = 0;
   int next_lane = 0;  // HFX_SCHED_GROUP: lanes taken in turn
+  // worker thread: builds admitted streams (their objects stay in `slots`) and tears released
+  // ones down (their objects move into the job)
+  struct Job {
+    Stream *build = nullptr;
+    std::unique_ptr<Stream> teardown;
+  };
+  std::thread worker;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<Job> jobs;
+  bool stopping = false;
   // GPU timestamps on the server clock: `epoch` was recorded on an idle stream and had
   // completed by host time epoch_s, so event e completed at or before
   // epoch_s + elapsed(epoch, e), late by at most epoch_err_s (the recording's round trip)
@@ -234,6 +259,50 @@ hfx_status launch_window(hfx_server *s, Stream &st, int lane_idx) {
   return HFX_OK;
 }
 
+// Tear-down of a released stream: its windows are retired, so only its last ingest copies
+// and kernels may still run; wait for them, then free (hipFree may wait for the device).
+void teardown_stream(hfx_server *s, Stream &st) {
+  (void)hipStreamSynchronize(s->ingest);
+  if (st.launched > 0 && !st.ev_done.empty()) (void)hipEventSynchronize(st.ev_done[(st.launched - 1) % st.g.n_pos]);
+  st.free_all();
+}
+
+void worker_main(hfx_server *s) {
+  (void)hipSetDevice(s->device);
+  for (;;) {
+    hfx_server::Job job;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv.wait(lk, [&] { return s->stopping || !s->jobs.empty(); });
+      if (s->jobs.empty()) return;  // stopping, nothing left
+      job = std::move(s->jobs.front());
+      s->jobs.pop_front();
+    }
+    if (job.build) {
+      Stream &st = *job.build;
+      const hfx_status rc = build_stream(s, st);
+      if (rc != HFX_OK) {
+        st.error = hfx_last_error();
+        (void)hipStreamSynchronize(s->capture);
+        st.free_all();
+      }
+      st.state.store(rc == HFX_OK ? 1 : 2, std::memory_order_release);
+    } else if (job.teardown) {
+      teardown_stream(s, *job.teardown);
+    }
+  }
+}
+
+void post_job(hfx_server *s, hfx_server::Job job) {
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->jobs.push_back(std::move(job));
+  }
+  s->cv.notify_one();
+}
+
+bool ready(const Stream &st) { return st.state.load(std::memory_order_acquire) == 1; }
+
 // Windows of these two streams can share a grouped launch: the same plan shape (their z,
 // wavelength, pitch, cameras and rings may differ); hfx_moments_group checks the same.
 bool same_group(const Stream &a, const Stream &b) {
@@ -291,21 +360,21 @@ hfx_status launch_group(hfx_server *s, const std::vector<Stream *> &members, int
 // plan shape, on the next lane.
 hfx_status schedule_group(hfx_server *s, int32_t &n) {
   for (;;) {
-    std::vector<std::pair<double, Stream *>> ready;
+    std::vector<std::pair<double, Stream *>> rdy;
     for (auto &up : s->slots) {
-      if (!up) continue;
+      if (!up || !ready(*up)) continue;
       Stream &st = *up;
-      if (st.launched < st.ready_windows()) ready.emplace_back(st.due_t[st.launched % st.g.n_pos], &st);
+      if (st.launched < st.ready_windows()) rdy.emplace_back(st.due_t[st.launched % st.g.n_pos], &st);
     }
-    if (ready.empty()) return HFX_OK;
-    std::stable_sort(ready.begin(), ready.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
-    std::vector<Stream *> members{ready[0].second};
+    if (rdy.empty()) return HFX_OK;
+    std::stable_sort(rdy.begin(), rdy.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+    std::vector<Stream *> members{rdy[0].second};
     // group size: four windows for frames up to 768^2 (their per-window cost drops by 30-40 %
     // in groups of four), two beyond (1024^2 gains nothing from four: the planes of four
     // windows no longer stay in the caches)
     const int cap = members[0]->g.npix <= 768 * 768 ? HFX_GROUP_MAX : std::min(2, HFX_GROUP_MAX);
-    for (size_t i = 1; i < ready.size() && (int)members.size() < cap; ++i)
-      if (same_group(*members[0], *ready[i].second)) members.push_back(ready[i].second);
+    for (size_t i = 1; i < rdy.size() && (int)members.size() < cap; ++i)
+      if (same_group(*members[0], *rdy[i].second)) members.push_back(rdy[i].second);
     const int lane = s->next_lane;
     s->next_lane = (s->next_lane + 1) % (int)s->lanes.size();
     SRV_TRY(launch_group(s, members, lane));
@@ -385,12 +454,22 @@ hfx_status hfx_server_create(const hfx_server_desc *d, hfx_server **out) {
   s->lane_load.assign(d->n_lanes, 0);
   s->lane_windows.resize(d->n_lanes);
   s->slots.resize(d->max_streams);
+  hfx_server *raw = s.get();
+  s->worker = std::thread(worker_main, raw);
   *out = s.release();
   return HFX_OK;
 }
 
 hfx_status hfx_server_destroy(hfx_server *s) {
   if (!s) return HFX_OK;
+  if (s->worker.joinable()) {  // the worker finishes the queued builds and tear-downs first
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->stopping = true;
+    }
+    s->cv.notify_all();
+    s->worker.join();
+  }
   (void)hipDeviceSynchronize();
   for (auto &st : s->slots)
     if (st) st->free_all();
@@ -417,27 +496,36 @@ hfx_status hfx_server_admit(hfx_server *s, const hfx_stream_desc *d, int32_t *st
   if (slot < 0) return fail(HFX_NOT_READY, "no free stream slot");
   size_t free_b = 0, total_b = 0;
   SRV_HIP(hipMemGetInfo(&free_b, &total_b));
+  // streams still being built have not allocated yet: their bytes are spoken for
+  int64_t building = 0;
+  for (auto &up : s->slots)
+    if (up && up->state.load(std::memory_order_acquire) == 0) building += up->g.total();
   const int64_t need = g.total();
   if (s->d.hbm_budget_bytes > 0) {
     if (s->bytes_held + need > s->d.hbm_budget_bytes) return fail(HFX_NOT_READY, "stream does not fit the HBM budget now");
   }
-  if ((int64_t)free_b - need < s->d.hbm_reserve_bytes) return fail(HFX_NOT_READY, "not enough free HBM now");
+  if ((int64_t)free_b - building - need < s->d.hbm_reserve_bytes) return fail(HFX_NOT_READY, "not enough free HBM now");
   auto st = std::make_unique<Stream>();
   st->id = slot;
   st->d = *d;
   st->g = g;
   st->lane = (int)(std::min_element(s->lane_load.begin(), s->lane_load.end()) - s->lane_load.begin());
-  const hfx_status rc = build_stream(s, *st);
-  if (rc != HFX_OK) {
-    const std::string msg = hfx_last_error();
-    (void)hipStreamSynchronize(s->capture);
-    st->free_all();
-    return fail(rc, msg);
-  }
   s->lane_load[st->lane]++;
   s->bytes_held += need;
+  Stream *raw = st.get();
   s->slots[slot] = std::move(st);
+  hfx_server::Job job;
+  job.build = raw;
+  post_job(s, std::move(job));
   *stream_id = slot;
+  return HFX_OK;
+}
+
+hfx_status hfx_server_stream_state(hfx_server *s, int32_t stream_id, int32_t *state) {
+  Stream *st = stream_of(s, stream_id);
+  if (!st || !state) return fail(HFX_E_INVALID, "unknown stream id or null argument");
+  *state = st->state.load(std::memory_order_acquire);
+  if (*state == 2) return fail(HFX_E_HIP, "stream set-up failed: " + st->error);
   return HFX_OK;
 }
 
@@ -455,6 +543,9 @@ hfx_status hfx_server_ingest(hfx_server *s, int32_t stream_id, int64_t n_frames)
 hfx_status hfx_server_ingest_due(hfx_server *s, int32_t stream_id, int64_t n_frames, double due_s) {
   Stream *st = stream_of(s, stream_id);
   if (!st) return fail(HFX_E_INVALID, "unknown stream id");
+  const int state = st->state.load(std::memory_order_acquire);
+  if (state == 0) return fail(HFX_NOT_READY, "stream still being set up");
+  if (state == 2) return fail(HFX_E_HIP, "stream set-up failed: " + st->error);
   const Geometry &g = st->g;
   if (n_frames < 0 || n_frames > g.R) return fail(HFX_E_INVALID, "n_frames must be in [0, ring frames]");
   if (n_frames == 0) return HFX_OK;
@@ -497,7 +588,7 @@ static hfx_status schedule_edf(hfx_server *s, int32_t &n) {
     Stream *best = nullptr;
     double best_deadline = 0.0;
     for (auto &up : s->slots) {
-      if (!up) continue;
+      if (!up || !ready(*up)) continue;
       Stream &st = *up;
       if (st.launched >= st.ready_windows()) continue;
       if (st.launched > 0 && window_on_gpu(st, st.launched - 1)) continue;  // one window in flight
@@ -526,15 +617,14 @@ hfx_status hfx_server_schedule(hfx_server *s, int32_t *launched) {
   } else {
     // every ready window, oldest camera due time first across the streams (a stream's own
     // windows keep their order: their due times do not decrease)
-    std::vector<std::pair<double, Stream *>> ready;
+    std::vector<std::pair<double, Stream *>> rdy;
     for (auto &up : s->slots) {
-      if (!up) continue;
+      if (!up || !ready(*up)) continue;
       Stream &st = *up;
-      for (int64_t w = st.launched; w < st.ready_windows(); ++w) ready.emplace_back(st.due_t[w % st.g.n_pos], &st);
+      for (int64_t w = st.launched; w < st.ready_windows(); ++w) rdy.emplace_back(st.due_t[w % st.g.n_pos], &st);
     }
-    std::stable_sort(ready.begin(), ready.end(),
-                     [](const auto &a, const auto &b) { return a.first < b.first; });
-    for (auto &r : ready) {
+    std::stable_sort(rdy.begin(), rdy.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+    for (auto &r : rdy) {
       SRV_TRY(launch_window(s, *r.second, r.second->lane));
       ++n;
     }
@@ -547,7 +637,7 @@ hfx_status hfx_server_poll(hfx_server *s, hfx_window_done *out, int32_t cap, int
   if (!s || !n || (cap > 0 && !out)) return fail(HFX_E_INVALID, "null argument");
   int32_t k = 0;
   for (auto &up : s->slots) {
-    if (!up) continue;
+    if (!up || !ready(*up)) continue;
     Stream &st = *up;
     while (k < cap && st.retired < st.launched) {
       const int64_t p = st.retired % st.g.n_pos;
@@ -576,7 +666,7 @@ hfx_status hfx_server_poll(hfx_server *s, hfx_window_done *out, int32_t cap, int
 
 hfx_status hfx_server_window_output(hfx_server *s, int32_t stream_id, int64_t window, const float **out) {
   Stream *st = stream_of(s, stream_id);
-  if (!st || !out) return fail(HFX_E_INVALID, "unknown stream id or null argument");
+  if (!st || !out || !ready(*st)) return fail(HFX_E_INVALID, "unknown or not set-up stream id, or null argument");
   if (window < 0 || window >= st->retired || window + st->g.n_pos <= st->launched)
     return fail(HFX_E_INVALID, "window not retired or its output slot was reused");
   *out = st->outs + (window % st->g.n_pos) * st->g.out_elems;
@@ -586,14 +676,15 @@ hfx_status hfx_server_window_output(hfx_server *s, int32_t stream_id, int64_t wi
 hfx_status hfx_server_release(hfx_server *s, int32_t stream_id) {
   Stream *st = stream_of(s, stream_id);
   if (!st) return fail(HFX_E_INVALID, "unknown stream id");
-  SRV_HIP(hipStreamSynchronize(s->ingest));
-  for (auto l : s->lanes) SRV_HIP(hipStreamSynchronize(l));  // EDF may have used any lane
   for (auto &q : s->lane_windows)
     q.erase(std::remove_if(q.begin(), q.end(), [&](const auto &e) { return e.first == stream_id; }), q.end());
   s->lane_load[st->lane]--;
   s->bytes_held -= st->g.total();
-  st->free_all();
-  s->slots[stream_id].reset();
+  // the worker waits for the stream's last copies and kernels and frees it; its jobs run in
+  // order, so a stream released while still being built is built first
+  hfx_server::Job job;
+  job.teardown = std::move(s->slots[stream_id]);
+  post_job(s, std::move(job));
   return HFX_OK;
 }
 
@@ -637,7 +728,7 @@ hfx_status hfx_serve_workload(const hfx_workload_desc *d, hfx_workload_result *r
   // counts.
   struct Live {
     int32_t id = -1;
-    bool admitted = false, released = false;
+    bool admitted = false, started = false, released = false;
     int64_t arrived = 0, retired = 0;
     double due_s = 0;  // due time of window `arrived` (server clock)
   };
@@ -676,20 +767,29 @@ hfx_status hfx_serve_workload(const hfx_workload_desc *d, hfx_workload_result *r
       SRV_TRY(rc);
       L.admitted = true;
       const double t_adm = now_s();
-      const hfx_plan_desc &p = d->streams[next_admit].plan;
-      L.due_s = t_adm + (d->arrivals == HFX_ARRIVALS_PERIODIC
-                             ? (double)(p.window / p.stride) / d->window_rate_hz[next_admit]
-                             : next_gap(next_admit));
       by_id[L.id] = next_admit;
       if (r->admit_s) r->admit_s[next_admit] = t_adm - t0;
       ++active;
       r->max_concurrent = std::max(r->max_concurrent, active);
       ++next_admit;
     }
+    // cameras start once their stream is set up (the server's worker builds it)
+    for (int i = 0; i < next_admit; ++i) {
+      Live &L = live[i];
+      if (!L.admitted || L.started || L.released) continue;
+      int32_t state = 0;
+      SRV_TRY(hfx_server_stream_state(srv, L.id, &state));
+      if (state != 1) continue;
+      L.started = true;
+      const hfx_plan_desc &p = d->streams[i].plan;
+      L.due_s = now_s() + (d->arrivals == HFX_ARRIVALS_PERIODIC
+                               ? (double)(p.window / p.stride) / d->window_rate_hz[i]
+                               : next_gap(i));
+    }
     // window arrivals: the camera has delivered the window's frames by its due time
     for (int i = 0; i < next_admit; ++i) {
       Live &L = live[i];
-      if (!L.admitted || L.released) continue;
+      if (!L.admitted || !L.started || L.released) continue;
       while (L.arrived < wps && L.due_s <= now_s()) {
         const hfx_plan_desc &p = d->streams[i].plan;
         const hfx_status rc = hfx_server_ingest_due(srv, L.id, L.arrived == 0 ? p.window : p.stride, L.due_s);
@@ -720,7 +820,11 @@ hfx_status hfx_serve_workload(const hfx_workload_desc *d, hfx_workload_result *r
         SRV_TRY(hfx_server_window_output(srv, w.stream_id, w.window, &src));
         Stream *st = stream_of(srv, w.stream_id);
         const size_t elems = (size_t)st->g.out_elems;
-        SRV_HIP(hipMemcpy(d->capture_out + (size_t)w.window * elems, src, elems * 4, hipMemcpyDeviceToHost));
+        // on a non-blocking stream: a legacy-stream copy would have to wait on the worker's
+        // graph capture, which HIP refuses
+        SRV_HIP(hipMemcpyAsync(d->capture_out + (size_t)w.window * elems, src, elems * 4, hipMemcpyDeviceToHost,
+                               srv->ingest));
+        SRV_HIP(hipStreamSynchronize(srv->ingest));
       }
       ++L.retired;
       ++r->windows_retired;

@@ -235,15 +235,28 @@ class Server:
         d = desc if desc is not None else server_desc()
         check(lib().hfx_server_create(C.byref(d), C.byref(self._h)))
 
-    def admit(self, spec: StreamSpec) -> Optional[int]:
-        """Stream id, or None when no slot or memory is free now."""
+    def admit(self, spec: StreamSpec, wait: bool = True, timeout_s: float = 60.0) -> Optional[int]:
+        """Stream id, or None when no slot or memory is free now.  The server sets the stream
+        up on its worker thread; with wait, returns once it is ready to take frames."""
+        import time
         sid = C.c_int32(-1)
         d = spec.to_desc()
         rc = lib().hfx_server_admit(self._h, C.byref(d), C.byref(sid))
         if rc == _lib.HFX_NOT_READY:
             return None
         check(rc)
+        t0 = time.monotonic()
+        while wait and self.state(sid.value) == 0:
+            if time.monotonic() - t0 > timeout_s:
+                raise TimeoutError("stream set-up did not finish")
+            time.sleep(1e-3)
         return sid.value
+
+    def state(self, sid: int) -> int:
+        """0 being set up, 1 ready (a failed set-up raises)."""
+        st = C.c_int32(0)
+        check(lib().hfx_server_stream_state(self._h, sid, C.byref(st)))
+        return st.value
 
     def clock(self) -> float:
         """Seconds on the server's clock."""

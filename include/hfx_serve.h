@@ -12,7 +12,9 @@
  * Flow: admit (by free HBM and a free slot) -> ingest frames (synthetic camera into the
  * ring; refused while it would overwrite frames of windows not yet retired) -> schedule
  * (launch every ready window of every stream) -> poll (retire finished windows in order,
- * with their frame-to-moment latency) -> release.
+ * with their frame-to-moment latency) -> release.  Setting a stream up and tearing it down
+ * (allocations, tables, graph capture, frees) run on a worker thread of the server, so the
+ * thread that serves windows never waits for them.
  *
  * Latency of a window = time from its due time (the camera time of its last frame, given to
  * hfx_server_ingest_due, or the host time of the hfx_server_ingest call) to the completion
@@ -21,7 +23,8 @@
  * hfx_server_info.clock_error_us, never early, so a latency is never under-reported.  Time a
  * window waits for ring space before it can be ingested counts, since its due time is kept.
  *
- * One host thread drives a server; calls on one server must not overlap.
+ * One host thread drives a server; calls on one server must not overlap (the server's own
+ * worker thread is internal).
  */
 #ifndef HFX_SERVE_H_
 #define HFX_SERVE_H_
@@ -102,8 +105,14 @@ hfx_status hfx_server_stream_bytes(const hfx_stream_desc *d, int64_t *bytes);
 hfx_status hfx_server_create(const hfx_server_desc *d, hfx_server **out);
 /* Waits for all work, frees every stream. */
 hfx_status hfx_server_destroy(hfx_server *s);
-/* HFX_NOT_READY when no slot is free or the stream does not fit the memory budget now. */
+/* HFX_NOT_READY when no slot is free or the stream does not fit the memory budget now.
+ * On HFX_OK the slot and memory are the stream's; its device state (allocations, plan tables,
+ * the camera's source pool, captured graphs) is built by the server's worker thread, so the
+ * call returns at once: the stream takes frames once hfx_server_stream_state reports 1. */
 hfx_status hfx_server_admit(hfx_server *s, const hfx_stream_desc *d, int32_t *stream_id);
+/* *state: 0 being set up, 1 ready, 2 set-up failed (the call then returns HFX_E_HIP with the
+ * reason; release the stream). */
+hfx_status hfx_server_stream_state(hfx_server *s, int32_t stream_id, int32_t *state);
 /* Seconds on the server's clock (host steady clock, zero at hfx_server_create). */
 hfx_status hfx_server_clock(hfx_server *s, double *t_s);
 /* The camera writes the next n_frames frames of the stream into its ring; HFX_NOT_READY
@@ -120,7 +129,8 @@ hfx_status hfx_server_poll(hfx_server *s, hfx_window_done *out, int32_t cap, int
 /* Device pointer to the moments of a retired window ([n_out][OH][OW] float); valid until
  * that stream ingests the frames of window + queue_windows + T/S. */
 hfx_status hfx_server_window_output(hfx_server *s, int32_t stream_id, int64_t window, const float **out);
-/* Waits for the stream's work and frees it (slot and memory return to admission). */
+/* Returns the slot at once; the worker thread waits for the stream's last copies and kernels
+ * and frees its memory (admission sees that memory once it is freed). */
 hfx_status hfx_server_release(hfx_server *s, int32_t stream_id);
 hfx_status hfx_server_info_get(hfx_server *s, hfx_server_info *info);
 

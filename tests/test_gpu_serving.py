@@ -18,7 +18,7 @@ SMALL = dict(sizes=[(512, 512), (768, 768), (512, 2048)], windows=[32, 64])
 def _eager(spec: serving.StreamSpec, n_windows: int) -> np.ndarray:
     """The stream's windows through hfx_moments on a ring rebuilt from its camera frames."""
     cam = spec.to_desc().camera
-    P = 4 * spec.window
+    P = spec.window                      # the server's synthetic camera plays T frames cyclically
     pool, _ = hfx.synth_frames(cam, 0, P)
     plan = hfx.MomentPlan(hfx.MomentConfig(spec.height, spec.width, spec.window, spec.stride,
                                            spec.bin_begin, spec.bin_end, spec.prop, z_m=spec.z_m,

@@ -23,7 +23,7 @@ def test_stream_bytes_counts_ring_pool_outputs_and_plan():
     N = 1024 * 1024
     R = 64 + 4 * 16
     n_pos = R // 16
-    expect = R * N * 2 + 4 * 64 * N * 2 + n_pos * 3 * N * 4 + (2 * 15 * N * 8 + N * 8 + (1 << 20)) + N * 8
+    expect = R * N * 2 + 64 * N * 2 + n_pos * 3 * N * 4 + (2 * 15 * N * 8 + N * 8 + (1 << 20)) + N * 8
     assert serving.stream_bytes(s) == expect
 
 
