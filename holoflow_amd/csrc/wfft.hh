@@ -244,12 +244,22 @@ template <int N> struct WaveFft {
   // tw: the TW-entry pass-major table from build_twiddles; inv: kernel exp(+...), unscaled.
   template <bool INV>
   __device__ __forceinline__ static void run(float2 (&v)[E], float2 *scr, const float2 *tw, int lane) {
+    stage_in<INV>(v, scr, lane);
+    stage_out<INV>(v, scr, tw, lane);
+  }
+  // The two halves of run(), for callers that load the next transform's input while this one
+  // runs: stage_in consumes v (pass 1, into the scratch), after which v may be reloaded;
+  // stage_out runs the remaining passes and leaves X in o.
+  template <bool INV> __device__ __forceinline__ static void stage_in(float2 (&v)[E], float2 *scr, int lane) {
     constexpr int S = INV ? 1 : -1;
     RegDft<E, S>::run(v);  // pass 1: butterfly j = lane over x[lane + 64 r]
 #pragma unroll
     for (int r = 0; r < E; ++r) scr[pad(lane * E + r)] = v[r];
     __builtin_amdgcn_wave_barrier();
-    pass<E, S>(v, scr, tw, lane);
+  }
+  template <bool INV>
+  __device__ __forceinline__ static void stage_out(float2 (&o)[E], float2 *scr, const float2 *tw, int lane) {
+    pass<E, INV ? 1 : -1>(o, scr, tw, lane);
   }
   // run-time direction (a wave-uniform branch between the two instantiations)
   __device__ __forceinline__ static void run(float2 (&v)[E], float2 *scr, const float2 *tw, int lane,

@@ -146,24 +146,19 @@ __global__ __launch_bounds__(64 * wrow_waves(N, KIND)) void k_wrow(PassArgs a) {
     constexpr int NACC = KIND == BLK_TO_MOM5 ? 5 : 3;
     Moments<E, NACC> acc;
     acc.zero();
-    if constexpr (E <= 16) {
-      float2 v[E], nxt[E];
-      load(nxt, 0);
+    {
+      // the next plane's row is loaded once pass 1 has moved this one into the scratch, so
+      // its latency overlaps the remaining passes (at 2048 points the wave holds 5*E sums, v,
+      // the output o and the pass temporaries: about 500 registers, one wave per SIMD)
+      float2 v[E];
+      load(v, 0);
       for (int p = 0; p < a.planes; ++p) {
         const float f = a.mom.f[p], f2 = a.mom.f2[p];
-#pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = nxt[m];
-        if (p + 1 < a.planes) load(nxt, p + 1);
-        F::template run<row_inv(KIND)>(v, ws, twl, lane);
-        acc.add(v, f, f2, p < a.mom.split, a.mom.power2);
-      }
-    } else {  // 2048-point rows: no register room for a prefetch next to 5*E sums
-      for (int p = 0; p < a.planes; ++p) {
-        const float f = a.mom.f[p], f2 = a.mom.f2[p];
-        float2 v[E];
-        load(v, p);
-        F::template run<row_inv(KIND)>(v, ws, twl, lane);
-        acc.add(v, f, f2, p < a.mom.split, a.mom.power2);
+        F::template stage_in<row_inv(KIND)>(v, ws, lane);
+        if (p + 1 < a.planes) load(v, p + 1);
+        float2 o[E];
+        F::template stage_out<row_inv(KIND)>(o, ws, twl, lane);
+        acc.add(o, f, f2, p < a.mom.split, a.mom.power2);
       }
     }
 #pragma unroll
@@ -210,10 +205,9 @@ __global__ __launch_bounds__(64 * wrow_waves(N, KIND)) void k_wrow(PassArgs a) {
 // 256 registers each) instead of three, which spilled.
 constexpr int wcol_subsets(int n, int kind) { return n >= 2048 ? 1 : kind == BLK_TO_MOM5 ? 2 : 3; }
 constexpr bool wcol_prefetch(int n, int kind) { return n >= 2048 ? false : wcol_subsets(n, kind) < 3; }
-// Planes per block of the plane-parallel column kinds: one (a register prefetch of the next
+// The plane-parallel column kinds take one plane per block: a register prefetch of the next
 // plane's column costs waves or spills; at 2048 rows two waves per SIMD hide the latency
-// instead, see wcol_tiles).
-constexpr int wcol_ppb(int, int) { return 1; }
+// instead (wcol_tiles).
 // Column tiles per workgroup of the plane-parallel kinds: two at 2048 rows (eight waves share
 // one twiddle table: 155 KB of LDS, two waves per SIMD instead of one), else one.
 constexpr int wcol_tiles(int n, int kind) { return n >= 2048 && !is_mom(kind) ? 2 : 1; }
@@ -241,7 +235,20 @@ __global__ __launch_bounds__(wcol_threads(N, KIND)) void k_wcol(PassArgs a) {
   // the 8 tiles that share an output cache line (32 floats) then share an L2 that merges
   // their 16-byte pieces before they go out
   // plane-parallel kinds with TL tiles: wave >> 2 picks the tile (sub is 0 for them)
-  const int bx = xcd_tile(blockIdx.x, gridDim.x) * TL + (TL > 1 ? (int)(wave >> 2) : 0);
+  // BLK_MID multiplies every plane's column by the same transfer-function tile, so its blocks
+  // run plane-fastest (block = tile * planes + plane, in contiguous runs per XCD): the planes
+  // of one tile are in flight together on one XCD and read the table tile from its L2,
+  // instead of the whole table streaming from memory once per plane.
+  int bt, bp;
+  if constexpr (KIND == BLK_MID) {
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    bt = b / a.planes;
+    bp = b - bt * a.planes;
+  } else {
+    bt = xcd_tile(blockIdx.x, gridDim.x);
+    bp = blockIdx.y;
+  }
+  const int bx = bt * TL + (TL > 1 ? (int)(wave >> 2) : 0);
   const int x = bx * C + c;
   const bool active = c < C && bx * C < a.W;
   const long long plane_elems = (long long)a.H * a.W;
@@ -344,23 +351,9 @@ __global__ __launch_bounds__(wcol_threads(N, KIND)) void k_wcol(PassArgs a) {
         }
       }
     };
-    constexpr int PPB = wcol_ppb(N, KIND);
-    if constexpr (PPB == 1) {
-      float2 v[E];
-      load(v, blockIdx.y);
-      finish(v, blockIdx.y);
-    } else {
-      const int p0 = blockIdx.y * PPB, p1 = min(p0 + PPB, a.planes);
-      float2 nxt[E];
-      load(nxt, p0);
-      for (int p = p0; p < p1; ++p) {
-        float2 v[E];
-#pragma unroll
-        for (int m = 0; m < E; ++m) v[m] = nxt[m];
-        if (p + 1 < p1) load(nxt, p + 1);
-        finish(v, p);
-      }
-    }
+    float2 v[E];
+    load(v, bp);
+    finish(v, bp);
   }
 }
 
@@ -384,8 +377,10 @@ template <int N, int KIND> static int launch_wrow_n(const PassArgs &a, hipStream
 template <int N, int KIND> static int launch_wcol_n(const PassArgs &a, hipStream_t st) {
   if (a.inv != 0) return -5;
   constexpr int TL = wcol_tiles(N, KIND);
-  const unsigned gx = (unsigned)((a.W / a.blk_c + TL - 1) / TL);
-  const unsigned gy = is_mom(KIND) ? 1u : (unsigned)((a.planes + wcol_ppb(N, KIND) - 1) / wcol_ppb(N, KIND));
+  const unsigned tiles = (unsigned)((a.W / a.blk_c + TL - 1) / TL);
+  // BLK_MID: one flat dimension, plane-fastest (see k_wcol); the plane-parallel others: y = plane
+  const unsigned gx = KIND == BLK_MID ? tiles * (unsigned)a.planes : tiles;
+  const unsigned gy = is_mom(KIND) || KIND == BLK_MID ? 1u : (unsigned)a.planes;
   const unsigned nt = wcol_threads(N, KIND);
   if (a.nwin > 1)
     hipLaunchKernelGGL((k_wcol<N, KIND, true>), dim3(gx, gy, (unsigned)a.nwin), dim3(nt), 0, st, a);
