@@ -216,6 +216,7 @@ hfx_status hfx_plan_create(const hfx_plan_desc *dp, hfx_stream_t stream, hfx_pla
     return fail(HFX_E_UNSUPPORTED, "16-bit MFMA time DFT needs a window multiple of 32");
   if (d.precision < 0 || d.precision > 3) return fail(HFX_E_INVALID, "unknown precision");
   if (d.width % 4 != 0) return fail(HFX_E_UNSUPPORTED, "frame width must be a multiple of 4");
+  if ((long long)d.height * d.width > INT32_MAX) return fail(HFX_E_UNSUPPORTED, "frame holds more than 2^31 - 1 pixels");
   if (!(d.lambda_m > 0)) return fail(HFX_E_INVALID, "wavelength must be positive");
   if (!(d.dx_m > 0) || !(d.dy_m > 0)) return fail(HFX_E_INVALID, "pixel pitch must be positive");
   if (d.dx_m != d.dy_m) return fail(HFX_E_INVALID, "dx must equal dy");
@@ -359,6 +360,7 @@ static hfx_status check_ring(const hfx_plan *p, const uint16_t *ring, int64_t ri
                              int64_t first) {
   if (!p || !ring) return fail(HFX_E_INVALID, "null argument");
   if (ring_frames < p->d.window) return fail(HFX_E_INVALID, "ring holds fewer frames than the window");
+  if (ring_frames > INT32_MAX) return fail(HFX_E_UNSUPPORTED, "ring holds more than 2^31 - 1 frames");
   if (first < 0 || first >= ring_frames) return fail(HFX_E_INVALID, "first_frame outside the ring");
   if ((reinterpret_cast<uintptr_t>(ring) & 7) != 0) return fail(HFX_E_INVALID, "ring must be 8-byte aligned");
   return HFX_OK;

@@ -94,38 +94,47 @@ __device__ __forceinline__ void dft_store_bin(const DftArgs &a, long long p0, in
   }
 }
 
+// Persistent: a block stages the operand matrix once (tens of KB, and index arithmetic per
+// element) and then walks groups of four 64-pixel tiles blockIdx, blockIdx + grid, ...
 template <int NBT, int MODE>
 __global__ __launch_bounds__(256) void k_dft(DftArgs a) {
   extern __shared__ float4 lds_dyn[];
   dft_stage_matrix<NBT, MODE>(lds_dyn, a.wm_hi, a.wm_lo, a.r.T, threadIdx.x, 256);
   __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const long long p0w = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
-  uint2 pre[1][dft_pre_steps<MODE>()];
-  dft_prefetch<MODE, 1>(a.r, p0w, lane, pre);
-  f4 acc1[1][NBT][4];
-  dft_tile<NBT, MODE, 1>(a.r, lds_dyn, p0w, lane, pre, acc1);
-  auto &acc = acc1[0];
-
-  const long long p0 = p0w + 4 * (lane & 15);
-  if (p0 >= a.r.npix) return;
-  float2 L[4];
+  const long long tiles = (a.r.npix + 63) / 64;
   const bool lens = a.lx != nullptr;
-  if (lens) {
-    const int y = (int)(p0 / a.W), x = (int)(p0 % a.W);  // W % 4 == 0: same row
-    const float2 ly = a.ly[y];
+  for (long long t0 = (long long)blockIdx.x * 4; t0 < tiles; t0 += (long long)gridDim.x * 4) {
+    // an opaque per-tile copy of the lane index keeps the lane-derived LDS and load addresses of
+    // the MFMA loop from being hoisted out of the tile loop (where they would hold registers)
+    int lane;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(lane) : "v"(threadIdx.x & 63));
+    // tiles past the end load clamped pixels (dft_px) and store nothing
+    const long long p0w = (t0 + (threadIdx.x >> 6)) * 64;
+    uint2 pre[1][dft_pre_steps<MODE>()];
+    dft_prefetch<MODE, 1>(a.r, p0w, lane, pre);
+    f4 acc1[1][NBT][4];
+    dft_tile<NBT, MODE, 1>(a.r, lds_dyn, p0w, lane, pre, acc1);
+    auto &acc = acc1[0];
+
+    const long long p0 = p0w + 4 * (lane & 15);
+    if (p0 >= a.r.npix) continue;
+    float2 L[4];
+    if (lens) {
+      const int y = (int)(p0 / a.W), x = (int)(p0 % a.W);  // W % 4 == 0: same row
+      const float2 ly = a.ly[y];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) L[t] = cmul(a.lx[x + t], ly);
-  }
+      for (int t = 0; t < 4; ++t) L[t] = cmul(a.lx[x + t], ly);
+    }
 #pragma unroll
-  for (int bt = 0; bt < NBT; ++bt) {
+    for (int bt = 0; bt < NBT; ++bt) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int bin = bt * 8 + 2 * (lane >> 4) + h;
-      if (bin >= a.nbc) continue;
-      float2 v[4];
-      dft_bin_values(a, acc[bt], h, bin, v);
-      dft_store_bin(a, p0, bin, v, L, lens);
+      for (int h = 0; h < 2; ++h) {
+        const int bin = bt * 8 + 2 * (lane >> 4) + h;
+        if (bin >= a.nbc) continue;
+        float2 v[4];
+        dft_bin_values(a, acc[bt], h, bin, v);
+        dft_store_bin(a, p0, bin, v, L, lens);
+      }
     }
   }
 }
@@ -133,46 +142,55 @@ __global__ __launch_bounds__(256) void k_dft(DftArgs a) {
 // FP32 stand-alone time DFT with the real-input symmetry folded (dft_fold_tile): NBF tiles of
 // 16 bins per wave, T % 32 == 0.  Same outputs as k_dft<NBT, 0> up to FP32 rounding, with
 // 2 (T/8) + 1 MFMAs per 16 bins and 16 pixels instead of T/2.  SPLIT waves share a 64-pixel
-// tile and take NBF tiles of bins each (their frame loads meet in the L1): with four tiles in
-// one wave the accumulators alone fill 128 registers and a SIMD runs one wave, two waves of
-// two tiles each leave room for three.
+// tile and take NBF tiles of bins each (their frame loads meet in the L1).
+// Persistent like k_dft.  The second launch bound (resident blocks per CU) makes the compiler
+// keep the accumulators in ordinary registers instead of splitting them off into accumulation
+// registers: up to two tiles per wave fit in 136 registers, three waves per SIMD (three tiles
+// take two blocks per CU, 256 registers, so that they do not spill).
+constexpr int fold_min_blocks(int nbf) { return nbf <= 2 ? 3 : 2; }
 template <int NBF, int SPLIT = 1>
-__global__ __launch_bounds__(256) void k_dft_fold(DftArgs a) {
+__global__ __launch_bounds__(256, fold_min_blocks(NBF)) void k_dft_fold(DftArgs a) {
   extern __shared__ float4 lds_dyn[];
   float *lf = reinterpret_cast<float *>(lds_dyn);
   stage_fold_matrix_rt(lf, reinterpret_cast<const float *>(a.wm_hi), a.r.T, NBF * SPLIT, a.nbc, threadIdx.x, 256);
   __syncthreads();
-  const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int part = SPLIT > 1 ? wave % SPLIT : 0;  // bin tiles part * NBF .. of this wave
-  const long long p0w = ((long long)blockIdx.x * (4 / SPLIT) + wave / SPLIT) * 64;
-  f4 ac[NBF][4], as[NBF][4];
-  dft_fold_tile<NBF>(a.r, lf, p0w, lane, ac, as, part * NBF, NBF * SPLIT);
-
-  const long long p0 = p0w + 4 * (lane & 15);
-  if (p0 >= a.r.npix) return;
-  float2 L[4];
+  constexpr int TPB = 4 / SPLIT;                   // pixel tiles per block iteration
+  const long long tiles = (a.r.npix + 63) / 64;
   const bool lens = a.lx != nullptr;
-  if (lens) {
-    const int y = (int)(p0 / a.W), x = (int)(p0 % a.W);  // W % 4 == 0: same row
-    const float2 ly = a.ly[y];
+  for (long long t0 = (long long)blockIdx.x * TPB; t0 < tiles; t0 += (long long)gridDim.x * TPB) {
+    int lane;  // opaque per tile, as in k_dft
+    asm volatile("v_mov_b32 %0, %1" : "=v"(lane) : "v"(threadIdx.x & 63));
+    // tiles past the end load clamped pixels (dft_px) and store nothing
+    const long long p0w = (t0 + wave / SPLIT) * 64;
+    f4 ac[NBF][4], as[NBF][4];
+    dft_fold_tile<NBF>(a.r, lf, p0w, lane, ac, as, part * NBF, NBF * SPLIT);
+
+    const long long p0 = p0w + 4 * (lane & 15);
+    if (p0 >= a.r.npix) continue;
+    float2 L[4];
+    if (lens) {
+      const int y = (int)(p0 / a.W), x = (int)(p0 % a.W);  // W % 4 == 0: same row
+      const float2 ly = a.ly[y];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) L[t] = cmul(a.lx[x + t], ly);
-  }
+      for (int t = 0; t < 4; ++t) L[t] = cmul(a.lx[x + t], ly);
+    }
 #pragma unroll
-  for (int f = 0; f < NBF; ++f) {
+    for (int f = 0; f < NBF; ++f) {
 #pragma unroll
-    for (int rr = 0; rr < 4; ++rr) {
-      const int bin = 16 * (part * NBF + f) + 4 * (lane >> 4) + rr;
-      if (bin >= a.nbc) continue;
-      float2 v[4];
+      for (int rr = 0; rr < 4; ++rr) {
+        const int bin = 16 * (part * NBF + f) + 4 * (lane >> 4) + rr;
+        if (bin >= a.nbc) continue;
+        float2 v[4];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) v[t] = make_float2(ac[f][t][rr], as[f][t][rr]);
-      if (dc_bin(a, bin)) {
+        for (int t = 0; t < 4; ++t) v[t] = make_float2(ac[f][t][rr], as[f][t][rr]);
+        if (dc_bin(a, bin)) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t) v[t].x += 2048.0f * (float)a.r.T;
+          for (int t = 0; t < 4; ++t) v[t].x += 2048.0f * (float)a.r.T;
+        }
+        dft_store_bin(a, p0, bin, v, L, lens);
       }
-      dft_store_bin(a, p0, bin, v, L, lens);
     }
   }
 }
@@ -377,13 +395,12 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
 
   // producers: the band spectra of row i for their 128 pixels
   auto dft_row = [&](int i, int lane) {
-    const bool has_next = i + 1 < nr;
-    const int j = has_next ? i + 1 : i;
+    const int j = i + 1 < nr ? i + 1 : i;  // the last row refills from itself (loads unused)
     if constexpr (FOLD)
-      stream.row(ring_of(i), ring_of(j), wmat, px0(row_y(i)), px0(row_y(j)), has_next, lane, fre, fim);
+      stream.row(ring_of(i), ring_of(j), wmat, px0(row_y(i)), px0(row_y(j)), lane, fre, fim);
     else
       stream.row(ring_of(i), ring_of(j), reinterpret_cast<const float *>(wmat), px0(row_y(i)), px0(row_y(j)),
-                 has_next, lane, acc);
+                 lane, acc);
   };
   // producers: MFMA registers of row i -> padded LDS rows (its window's lens for Fresnel)
   auto scatter = [&](int i, int lane) {
@@ -488,6 +505,7 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
       if (i + 1 < nr) scatter(i + 1, lane);
       __syncthreads();  // row i + 1's planes are in LDS
     }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the last row's unused refills land first
   } else {
     __syncthreads();
     for (int i = 0; i < nr; ++i) {
@@ -536,10 +554,22 @@ static DftArgs make_args(const DftLaunch &L) {
   return a;
 }
 
+// Grid of a persistent stand-alone DFT kernel: as many blocks as are resident at once (the
+// occupancy calculator's count per CU times the CUs), at most one per work item.
+static unsigned persistent_grid(const void *kernel, size_t lds, long long items) {
+  int dev = 0, cus = 256, per_cu = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, lds) != hipSuccess || per_cu < 1) per_cu = 1;
+  const long long g = (long long)cus * per_cu;
+  return (unsigned)(items < g ? (items > 0 ? items : 1) : g);
+}
+
 template <int NBT, int MODE> static int launch_plain(const DftArgs &a, hipStream_t st) {
   const long long waves = (a.r.npix + 63) / 64;
   const size_t lds = dft_matrix_lds_bytes(NBT, MODE, a.r.T);
-  hipLaunchKernelGGL((k_dft<NBT, MODE>), dim3((unsigned)((waves + 3) / 4)), dim3(256), lds, st, a);
+  const unsigned grid = persistent_grid(reinterpret_cast<const void *>(&k_dft<NBT, MODE>), lds, (waves + 3) / 4);
+  hipLaunchKernelGGL((k_dft<NBT, MODE>), dim3(grid), dim3(256), lds, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -555,8 +585,10 @@ template <int NBT> static int launch_plain_mode(const DftArgs &a, int mode, hipS
 
 template <int NBF, int SPLIT = 1> static int launch_fold(const DftArgs &a, hipStream_t st) {
   const long long waves = (a.r.npix + 63) / 64 * SPLIT;
-  hipLaunchKernelGGL((k_dft_fold<NBF, SPLIT>), dim3((unsigned)((waves + 3) / 4)), dim3(256),
-                     dft_fold_lds_bytes(NBF * SPLIT, a.r.T), st, a);
+  const size_t lds = dft_fold_lds_bytes(NBF * SPLIT, a.r.T);
+  const unsigned grid =
+      persistent_grid(reinterpret_cast<const void *>(&k_dft_fold<NBF, SPLIT>), lds, (waves + 3) / 4);
+  hipLaunchKernelGGL((k_dft_fold<NBF, SPLIT>), dim3(grid), dim3(256), lds, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -23,6 +23,8 @@
 // bt*8 + 2*(lane>>4) + {0, 1}.
 #pragma once
 
+#include <type_traits>
+
 #include "hfx_device.hh"
 
 namespace hfx {
@@ -42,16 +44,26 @@ struct DftRing {
 // Frame loads are non-temporal: a window's samples are read once per launch, and keeping them
 // out of the caches leaves the last-level cache to the band planes that the column pass reads
 // back right after (whole-batch throughput +3 % at 1024^2, T = 64; results unchanged).
-// BC: pixels past the end (a ragged last tile) read 2048, a centred zero.  Callers whose pixels
-// are known to be in range (the rows of the fused kernels) pass BC = false: the check becomes a
-// branch around every load, which splits a step loop into basic blocks and makes a step wait
-// for every load in flight instead of its own (front kernel 92 -> 86 us at 1024^2, T = 64).
-template <bool BC = true>
-__device__ __forceinline__ uint2 dft_load4(const DftRing &r, long long frame, long long p) {
-  if (BC && p >= r.npix) return make_uint2(0x08000800u, 0x08000800u);
+// Loads carry no bounds check: a check is a branch around every load, which splits a step loop
+// into basic blocks and makes a step wait for every load in flight instead of its own (front
+// kernel 92 -> 86 us at 1024^2, T = 64).  Callers clamp their pixel indices instead
+// (dft_px): pixels past the end of a ragged last tile read the frame's last pixels, and the
+// caller stores no result for them (the MFMA output column of a pixel depends on that pixel's
+// inputs alone).
+// Frame and pixel counts are below 2^31 (the plan and ring checks enforce it): the element
+// offset is one 32 x 32 -> 64-bit multiply-add, where 64-bit index arithmetic costs about
+// three times the VALU instructions in the hottest loop of the time DFT.
+__device__ __forceinline__ uint2 dft_load4(const DftRing &r, int frame, long long p) {
   typedef unsigned u2v __attribute__((ext_vector_type(2)));
-  const u2v v = __builtin_nontemporal_load(reinterpret_cast<const u2v *>(r.ring + frame * r.npix + p));
+  const unsigned long long off = (unsigned long long)(unsigned)frame * (unsigned)r.npix + (unsigned long long)p;
+  const u2v v = __builtin_nontemporal_load(reinterpret_cast<const u2v *>(r.ring + off));
   return make_uint2(v.x, v.y);
+}
+
+// First of the 4 pixels a lane loads in tile i of a wave's tiles at p0w, clamped into the frame
+__device__ __forceinline__ long long dft_px(const DftRing &r, long long p0w, int lane, int i) {
+  const long long p = p0w + 64 * i + 4 * (lane & 15);
+  return p < r.npix ? p : r.npix - 4;
 }
 
 // Stages the operand matrix into LDS.  FP32 rows (MODE 0) use a pitch of T+4 floats, so the
@@ -85,9 +97,9 @@ __host__ __device__ constexpr size_t dft_matrix_lds_bytes(int nbt, int mode, int
 // frames; MODE 1..3: one step of 32), for each of its TPW tiles.
 template <int MODE> constexpr int dft_pre_steps() { return 8; }
 
-__device__ __forceinline__ long long ring_frame(const DftRing &r, long long n) {
-  long long f = r.first + n;
-  return f >= r.ring_frames ? f - r.ring_frames : f;  // n < T <= ring_frames
+__device__ __forceinline__ int ring_frame(const DftRing &r, int n) {
+  const int f = (int)r.first + n, R = (int)r.ring_frames;
+  return f >= R ? f - R : f;  // n < T <= ring_frames
 }
 
 // The wave owns TPW adjacent 64-pixel tiles starting at p0w (a multiple of 4); tile i
@@ -98,22 +110,24 @@ __device__ __forceinline__ void dft_prefetch(const DftRing &r, long long p0w, in
                                              uint2 (&pre)[TPW][PRE]) {
   static_assert(MODE == 0 || PRE == 8, "16-bit modes prefetch one 32-frame step");
   const int sub = lane >> 4;
-  const long long p0 = p0w + 4 * (lane & 15);
+  long long p0[TPW];  // clamped (dft_px)
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) p0[i] = dft_px(r, p0w, lane, i);
   if (MODE == 0) {
 #pragma unroll
     for (int s = 0; s < PRE; ++s) {
       if (4 * s < r.T) {
-        const long long f = ring_frame(r, 4 * s + sub);
+        const int f = ring_frame(r, 4 * s + sub);
 #pragma unroll
-        for (int i = 0; i < TPW; ++i) pre[i][s] = dft_load4(r, f, p0 + 64 * i);
+        for (int i = 0; i < TPW; ++i) pre[i][s] = dft_load4(r, f, p0[i]);
       }
     }
   } else {
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      const long long f = ring_frame(r, 8 * sub + q);
+      const int f = ring_frame(r, 8 * sub + q);
 #pragma unroll
-      for (int i = 0; i < TPW; ++i) pre[i][q] = dft_load4(r, f, p0 + 64 * i);
+      for (int i = 0; i < TPW; ++i) pre[i][q] = dft_load4(r, f, p0[i]);
     }
   }
 }
@@ -133,7 +147,9 @@ __device__ __forceinline__ void dft_tile(const DftRing &r, const void *lds, long
                                          f4 (&acc)[TPW][NBT][4]) {
   const int T = r.T;
   const int sub = lane >> 4;
-  const long long p0 = p0w + 4 * (lane & 15);
+  long long p0[TPW];  // clamped (dft_px)
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) p0[i] = dft_px(r, p0w, lane, i);
 #pragma unroll
   for (int i = 0; i < TPW; ++i)
 #pragma unroll
@@ -167,9 +183,9 @@ __device__ __forceinline__ void dft_tile(const DftRing &r, const void *lds, long
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         if (s0 + k < steps) {
-          const long long f = ring_frame(r, 4 * (s0 + k) + sub);
+          const int f = ring_frame(r, 4 * (s0 + k) + sub);
 #pragma unroll
-          for (int i = 0; i < TPW; ++i) nxt[k][i] = dft_load4(r, f, p0 + 64 * i);
+          for (int i = 0; i < TPW; ++i) nxt[k][i] = dft_load4(r, f, p0[i]);
         }
       }
     };
@@ -202,9 +218,9 @@ __device__ __forceinline__ void dft_tile(const DftRing &r, const void *lds, long
     auto load_step = [&](int s) {
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const long long f = ring_frame(r, 32 * s + 8 * sub + q);
+        const int f = ring_frame(r, 32 * s + 8 * sub + q);
 #pragma unroll
-        for (int i = 0; i < TPW; ++i) nxt[i][q] = dft_load4(r, f, p0 + 64 * i);
+        for (int i = 0; i < TPW; ++i) nxt[i][q] = dft_load4(r, f, p0[i]);
       }
     };
     for (int s = 0; s < T / 32; ++s) {
@@ -276,10 +292,10 @@ template <int NBT, int TPW, int T, int K> struct DftStream {
   uint2 ring[K][TPW];
 
   __device__ __forceinline__ void load(const DftRing &r, long long p0w, int lane, int s, int slot) {
-    const long long f = ring_frame(r, 4 * s + (lane >> 4));
+    const int f = ring_frame(r, 4 * s + (lane >> 4));
     const long long p0 = p0w + 4 * (lane & 15);
 #pragma unroll
-    for (int i = 0; i < TPW; ++i) ring[slot][i] = dft_load4<false>(r, f, p0 + 64 * i);
+    for (int i = 0; i < TPW; ++i) ring[slot][i] = dft_load4(r, f, p0 + 64 * i);
   }
   // steps 0 .. K-1 of the first row
   __device__ __forceinline__ void start(const DftRing &r, long long p0w, int lane) {
@@ -287,10 +303,12 @@ template <int NBT, int TPW, int T, int K> struct DftStream {
     for (int k = 0; k < K; ++k) load(r, p0w, lane, k, k);
   }
   // D = Wm * X for the row at p0w of window r (operand matrix lw with pitch T + 4 floats);
-  // the ring refills from this row, then from the row at p0w_next of window r_next when
-  // has_next.
+  // the ring refills from this row, then from the row at p0w_next of window r_next.  On the
+  // last row the caller passes the row itself as the next one: those refills are never used,
+  // and issuing them anyway keeps every load out of branches (a conditional load makes the
+  // steps after it wait for all loads in flight instead of their own).
   __device__ __forceinline__ void row(const DftRing &r, const DftRing &r_next, const float *lw, long long p0w,
-                                      long long p0w_next, bool has_next, int lane, f4 (&acc)[TPW][NBT][4]) {
+                                      long long p0w_next, int lane, f4 (&acc)[TPW][NBT][4]) {
 #pragma unroll
     for (int i = 0; i < TPW; ++i)
 #pragma unroll
@@ -306,8 +324,9 @@ template <int NBT, int TPW, int T, int K> struct DftStream {
       for (int i = 0; i < TPW; ++i) unpack4(ring[slot][i], x[i]);
       if (s + K < STEPS)
         load(r, p0w, lane, s + K, slot);
-      else if (has_next)
+      else  // unconditional: see DftStream::row
         load(r_next, p0w_next, lane, s + K - STEPS, slot);
+      __builtin_amdgcn_sched_barrier(0);  // the refill is issued here, not sunk to its use
       const int n = 4 * s + (lane >> 4);
 #pragma unroll
       for (int bt = 0; bt < NBT; ++bt) {
@@ -343,21 +362,21 @@ template <int TPW, int T, int K> struct DftStreamFold {
 
   __device__ __forceinline__ void load(const DftRing &r, long long p0w, int lane, int s, int slot) {
     const int p = 4 * s + (lane >> 4) + 1;
-    const long long f1 = ring_frame(r, p), f2 = ring_frame(r, T - p);
+    const int f1 = ring_frame(r, p), f2 = ring_frame(r, T - p);
     const long long p0 = p0w + 4 * (lane & 15);
 #pragma unroll
     for (int i = 0; i < TPW; ++i) {
-      ring[slot][i][0] = dft_load4<false>(r, f1, p0 + 64 * i);
-      ring[slot][i][1] = dft_load4<false>(r, f2, p0 + 64 * i);
+      ring[slot][i][0] = dft_load4(r, f1, p0 + 64 * i);
+      ring[slot][i][1] = dft_load4(r, f2, p0 + 64 * i);
     }
   }
   __device__ __forceinline__ void start(const DftRing &r, long long p0w, int lane) {
 #pragma unroll
     for (int k = 0; k < K; ++k) load(r, p0w, lane, k, k);
   }
+  // (refills as in DftStream::row)
   __device__ __forceinline__ void row(const DftRing &r, const DftRing &r_next, const void *lds, long long p0w,
-                                      long long p0w_next, bool has_next, int lane, f4 (&ac)[TPW][4],
-                                      f4 (&as)[TPW][4]) {
+                                      long long p0w_next, int lane, f4 (&ac)[TPW][4], f4 (&as)[TPW][4]) {
     const float *lc = reinterpret_cast<const float *>(lds), *ls = lc + 16 * PC, *lx = ls + 16 * PC;
     const int g = lane >> 4, rw = lane & 15;
 #pragma unroll
@@ -370,10 +389,10 @@ template <int TPW, int T, int K> struct DftStreamFold {
     // x_0 (g = 0) and x_{T/2} (g >= 1; zero weights for g >= 2), used after the pair steps
     uint2 xe[TPW];
     {
-      const long long fe = ring_frame(r, g == 0 ? 0 : T / 2);
+      const int fe = ring_frame(r, g == 0 ? 0 : T / 2);
       const long long p0 = p0w + 4 * (lane & 15);
 #pragma unroll
-      for (int i = 0; i < TPW; ++i) xe[i] = dft_load4<false>(r, fe, p0 + 64 * i);
+      for (int i = 0; i < TPW; ++i) xe[i] = dft_load4(r, fe, p0 + 64 * i);
     }
 #pragma unroll
     for (int s = 0; s < STEPS; ++s) {
@@ -392,8 +411,9 @@ template <int TPW, int T, int K> struct DftStreamFold {
       }
       if (s + K < STEPS)
         load(r, p0w, lane, s + K, slot);
-      else if (has_next)
+      else  // unconditional: see DftStream::row
         load(r_next, p0w_next, lane, s + K - STEPS, slot);
+      __builtin_amdgcn_sched_barrier(0);  // the refill is issued here, not sunk to its use
       const float wc = lc[rw * PC + 4 * s + g], wsn = ls[rw * PC + 4 * s + g];
 #pragma unroll
       for (int i = 0; i < TPW; ++i)
@@ -425,7 +445,7 @@ __device__ __forceinline__ void dft_fold_tile(const DftRing &r, const float *lds
   const int T = r.T, steps = T / 8, PC = T / 2 + 4;
   const float *lc = lds + 16 * f0 * PC, *ls = lds + 16 * (nt + f0) * PC, *lx = lds + 32 * nt * PC + 64 * f0;
   const int g = lane >> 4, rw = lane & 15;
-  const long long p0 = p0w + 4 * (lane & 15);
+  const long long p0 = dft_px(r, p0w, lane, 0);
 #pragma unroll
   for (int f = 0; f < NBF; ++f)
 #pragma unroll
@@ -442,30 +462,41 @@ __device__ __forceinline__ void dft_fold_tile(const DftRing &r, const float *lds
   };
 #pragma unroll
   for (int k = 0; k < K; ++k) load(k, k);
-  for (int s0 = 0; s0 < steps; s0 += K) {
+  // step st from ring slot k, refilling the slot with step st + K when load_next is true_type;
+  // the last batch is peeled so that no load sits behind a branch (a conditional load makes
+  // every step wait for all loads in flight instead of its own)
+  auto step = [&](int st, int k, auto load_next) {
+    float x1[4], x2[4], sa[4], sb[4];
+    unpack4(ring[k][0], x1);
+    unpack4(ring[k][1], x2);
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int st = s0 + k;
-      float x1[4], x2[4], sa[4], sb[4];
-      unpack4(ring[k][0], x1);
-      unpack4(ring[k][1], x2);
+    for (int t = 0; t < 4; ++t) {
+      sa[t] = x1[t] + x2[t];
+      sb[t] = x1[t] - x2[t];
+    }
+    if constexpr (decltype(load_next)::value) {
+      load(st + K, k);
+      // issue the refill here: left to itself the scheduler sinks all refills of a batch to
+      // its end, right before their data is needed
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int f = 0; f < NBF; ++f) {
+      const float wc = lc[(16 * f + rw) * PC + 4 * st + g], wsn = ls[(16 * f + rw) * PC + 4 * st + g];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        sa[t] = x1[t] + x2[t];
-        sb[t] = x1[t] - x2[t];
-      }
-      if (st + K < steps) load(st + K, k);
-#pragma unroll
-      for (int f = 0; f < NBF; ++f) {
-        const float wc = lc[(16 * f + rw) * PC + 4 * st + g], wsn = ls[(16 * f + rw) * PC + 4 * st + g];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          ac[f][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(wc, sa[t], ac[f][t], 0, 0, 0);
-          as[f][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(wsn, sb[t], as[f][t], 0, 0, 0);
-        }
+        ac[f][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(wc, sa[t], ac[f][t], 0, 0, 0);
+        as[f][t] = __builtin_amdgcn_mfma_f32_16x16x4f32(wsn, sb[t], as[f][t], 0, 0, 0);
       }
     }
+  };
+  int s0 = 0;
+  for (; s0 + K < steps; s0 += K) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) step(s0 + k, k, std::true_type{});
   }
+#pragma unroll
+  for (int k = 0; k < K; ++k) step(s0 + k, k, std::false_type{});
   float x[4];
   unpack4(xe, x);
 #pragma unroll
