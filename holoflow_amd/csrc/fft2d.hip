@@ -193,7 +193,7 @@ __global__ __launch_bounds__((Geo<N, COLS>::NT)) void k_pass(PassArgs a) {
         if (y < a.H && x >= 0) {
           float2 v = lds[li];
           if constexpr (MOM) {
-            float m = v.x * v.x + v.y * v.y;
+            float m = sq_mag(v);
             if (!a.mom.power2) m = mag_sqrt(m);
             acc[0][it] += m;
             acc[1][it] += __fmul_rn(f, m);

@@ -153,6 +153,10 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
   return (b & 7) * (n8 >> 3) + (b >> 3);
 }
 
+// |v|^2 for the moment sums with its rounding pinned (one product, one fused multiply-add), so
+// that every pass and build forms the sums from the same values
+__device__ __forceinline__ float sq_mag(float2 v) { return __fmaf_rn(v.x, v.x, __fmul_rn(v.y, v.y)); }
+
 // |X| for the moment sums: the hardware square root (v_sqrt_f32, within 1 ulp) instead of the
 // correctly rounded sqrtf sequence, which costs about ten more VALU instructions per element in
 // the column pass; the moments are sums over planes held to a relative tolerance, not bits.

@@ -9,6 +9,8 @@
 //                 workgroups hold S plane subsets x 4 columns (S = 2, or 1 for 2048 rows);
 //                 the subsets' partial moment sums are added in a fixed order at the
 //                 end, so results are deterministic.
+#include <type_traits>
+
 #include "hfx_kernels.hh"
 #include "wfft.hh"
 
@@ -38,7 +40,7 @@ template <int E, int NACC> struct Moments {
   __device__ __forceinline__ void add(const float2 (&v)[E], float f, float f2, bool lo, bool power2) {
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      float a = v[m].x * v[m].x + v[m].y * v[m].y;
+      float a = sq_mag(v[m]);
       if (!power2) a = mag_sqrt(a);
       s[0][m] += a;
       s[1][m] += __fmul_rn(f, a);
@@ -49,6 +51,19 @@ template <int E, int NACC> struct Moments {
         else
           s[4][m] += a;
       }
+    }
+  }
+  // the three moment sums, plus sum 3 (NACC == 4) when HI: for the row pass's band split
+  template <bool HI>
+  __device__ __forceinline__ void add_split(const float2 (&v)[E], float f, float f2, bool power2) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      float a = sq_mag(v[m]);
+      if (!power2) a = mag_sqrt(a);
+      s[0][m] += a;
+      s[1][m] += __fmul_rn(f, a);
+      s[2][m] += __fmul_rn(f2, a);
+      if constexpr (HI) s[3][m] += a;
     }
   }
 };
@@ -71,8 +86,8 @@ template <bool G> __device__ __forceinline__ PassWin pass_window(const PassArgs 
   }
 }
 
-__device__ __forceinline__ void store_moment(const PassArgs &a, float *const (&mo)[5], int nacc, int y, int x,
-                                             const float (&val)[5]) {
+// offset of pixel (y, x) in a moment image (fftshift applied on store when asked)
+__device__ __forceinline__ long long moment_index(const PassArgs &a, int y, int x) {
   int ys = y, xs = x;  // 0 <= y < H, 0 <= x < W
   if (a.mom.shift) {   // (y + H - H/2) mod H, without a division
     ys = y + (a.H - a.H / 2);
@@ -80,7 +95,12 @@ __device__ __forceinline__ void store_moment(const PassArgs &a, float *const (&m
     xs = x + (a.W - a.W / 2);
     if (xs >= a.W) xs -= a.W;
   }
-  const long long o = (long long)ys * a.W + xs;
+  return (long long)ys * a.W + xs;
+}
+
+__device__ __forceinline__ void store_moment(const PassArgs &a, float *const (&mo)[5], int nacc, int y, int x,
+                                             const float (&val)[5]) {
+  const long long o = moment_index(a, y, x);
   const float n_all = (float)a.planes;
   mo[0][o] = val[0] / n_all;
   mo[1][o] = val[1] / n_all;
@@ -101,7 +121,9 @@ __device__ __forceinline__ void store_moment(const PassArgs &a, float *const (&m
 // keep four: their 5*E sums leave no register room for a second wave per SIMD.
 constexpr int wrow_waves(int n, int kind) { return n >= 2048 && !is_mom(kind) ? 8 : 4; }
 
-template <int N, int KIND, bool G = false>
+// PLAIN (NAT_TO_BLK only): the input is a complex plane of the output's size with no lens,
+// so a row loads from one base address with no per-element bounds checks or index products.
+template <int N, int KIND, bool G = false, bool PLAIN = false>
 __global__ __launch_bounds__(64 * wrow_waves(N, KIND)) void k_wrow(PassArgs a) {
   using F = WaveFft<N>;
   constexpr int E = F::E;
@@ -121,6 +143,12 @@ __global__ __launch_bounds__(64 * wrow_waves(N, KIND)) void k_wrow(PassArgs a) {
   float2 *ws = scr + wave * F::SCR;
 
   auto load = [&](float2 (&v)[E], long long p) {
+    if constexpr (PLAIN) {
+      const float2 *row = reinterpret_cast<const float2 *>(pw.in) + p * plane_elems + (long long)y * N;
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = row[lane + 64 * m];
+      return;
+    }
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int x = lane + 64 * m;
@@ -143,44 +171,71 @@ __global__ __launch_bounds__(64 * wrow_waves(N, KIND)) void k_wrow(PassArgs a) {
   };
 
   if constexpr (is_mom(KIND)) {
-    constexpr int NACC = KIND == BLK_TO_MOM5 ? 5 : 3;
-    Moments<E, NACC> acc;
-    acc.zero();
-    {
-      // the next plane's row is loaded once pass 1 has moved this one into the scratch, so
-      // its latency overlaps the remaining passes (at 2048 points the wave holds 5*E sums, v,
-      // the output o and the pass temporaries: about 500 registers, one wave per SIMD)
-      float2 v[E];
-      load(v, 0);
-      for (int p = 0; p < a.planes; ++p) {
+    // Sums over the planes of the row.  The band-split kind also needs M0 over the planes below
+    // the split and over those above it: the planes below run first with the three sums, M0
+    // low is stored from sum 0 at the split (the same additions in the same order as a sum of
+    // its own), then the planes above run with a fourth sum for M0 high.  Fewer sums are live
+    // at a time, and at 2048 points fewer of them spill into accumulation registers.
+    constexpr bool BAND = KIND == BLK_TO_MOM5;
+    Moments<E, BAND ? 4 : 3> acc;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int m = 0; m < E; ++m) acc.s[k][m] = 0.f;
+    // the next plane's row is loaded once pass 1 has moved this one into the scratch, so its
+    // latency overlaps the remaining passes; the load is unconditional (the last one reloads
+    // the last plane, unused), which keeps it out of a branch
+    float2 v[E];
+    load(v, 0);
+    auto run_planes = [&](int p0, int p1, auto hi) {
+      for (int p = p0; p < p1; ++p) {
         const float f = a.mom.f[p], f2 = a.mom.f2[p];
         F::template stage_in<row_inv(KIND)>(v, ws, lane);
-        if (p + 1 < a.planes) load(v, p + 1);
+        load(v, p + 1 < a.planes ? p + 1 : p);
         float2 o[E];
         F::template stage_out<row_inv(KIND)>(o, ws, twl, lane);
-        acc.add(o, f, f2, p < a.mom.split, a.mom.power2);
+        acc.template add_split<decltype(hi)::value>(o, f, f2, a.mom.power2);
       }
+    };
+    if constexpr (BAND) {
+      run_planes(0, a.mom.split, std::false_type{});
+      if (pw.mom_out[3]) {
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+          pw.mom_out[3][moment_index(a, y, lane + 64 * m)] = acc.s[0][m] / (float)a.mom.split;
+      }
+#pragma unroll
+      for (int m = 0; m < E; ++m) acc.s[3][m] = 0.f;
+      run_planes(a.mom.split, a.planes, std::true_type{});
+    } else {
+      run_planes(0, a.planes, std::false_type{});
     }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the unused last refill lands first
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      float val[5] = {acc.s[0][m], acc.s[1][m], acc.s[2][m], 0.f, 0.f};
-      if constexpr (NACC == 5) {
-        val[3] = acc.s[3][m];
-        val[4] = acc.s[4][m];
+      const long long o = moment_index(a, y, lane + 64 * m);
+      const float n_all = (float)a.planes;
+      pw.mom_out[0][o] = acc.s[0][m] / n_all;
+      pw.mom_out[1][o] = acc.s[1][m] / n_all;
+      pw.mom_out[2][o] = acc.s[2][m] / n_all;
+      if constexpr (BAND) {
+        if (pw.mom_out[4]) pw.mom_out[4][o] = acc.s[3][m] / (float)(a.planes - a.mom.split);
       }
-      store_moment(a, pw.mom_out, NACC, y, lane + 64 * m, val);
     }
   } else {
     const long long p = blockIdx.y;
     float2 v[E];
     load(v, p);
     F::template run<row_inv(KIND)>(v, ws, twl, lane);
+    // blocked store: element m of the lane's row at dst + m * mstep, the step a wave-uniform scalar
+    float2 *dst = reinterpret_cast<float2 *>(pw.out) + p * plane_elems + boff;
+    const long long mstep = 64LL * a.H;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int x = lane + 64 * m;
       float2 t = v[m];
       if (KIND == NAT_TO_BLK) {
-        reinterpret_cast<float2 *>(pw.out)[p * plane_elems + boff + (long long)m * 64 * a.H] = t;
+        dst[m * mstep] = t;
       } else {
         const long long o = p * plane_elems + (long long)y * a.W + x;
         if (a.out_mode == OUT_NAT_MAG) {
@@ -367,8 +422,12 @@ template <int N, int KIND> static int launch_wrow_n(const PassArgs &a, hipStream
   constexpr int RW = wrow_waves(N, KIND);
   const unsigned gx = (unsigned)((a.H + RW - 1) / RW);
   const unsigned gy = is_mom(KIND) ? 1u : (unsigned)a.planes;
+  const bool plain = KIND == NAT_TO_BLK && a.in_mode == IN_NAT_C && a.in_h == a.H && a.in_w == a.W &&
+                     a.off_x == 0 && a.off_y == 0 && a.pre_lx == nullptr;
   if (a.nwin > 1)
     hipLaunchKernelGGL((k_wrow<N, KIND, true>), dim3(gx, gy, (unsigned)a.nwin), dim3(64 * RW), 0, st, a);
+  else if (plain)
+    hipLaunchKernelGGL((k_wrow<N, KIND, false, KIND == NAT_TO_BLK>), dim3(gx, gy), dim3(64 * RW), 0, st, a);
   else
     hipLaunchKernelGGL((k_wrow<N, KIND>), dim3(gx, gy), dim3(64 * RW), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
