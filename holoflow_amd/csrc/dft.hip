@@ -140,32 +140,29 @@ __global__ __launch_bounds__(256) void k_dft(DftArgs a) {
 }
 
 // FP32 stand-alone time DFT with the real-input symmetry folded (dft_fold_tile): NBF tiles of
-// 16 bins per wave, T % 32 == 0.  Same outputs as k_dft<NBT, 0> up to FP32 rounding, with
-// 2 (T/8) + 1 MFMAs per 16 bins and 16 pixels instead of T/2.  SPLIT waves share a 64-pixel
-// tile and take NBF tiles of bins each (their frame loads meet in the L1).
+// 16 bins per wave (all of the band's tiles: every frame sample is loaded by one wave only),
+// T % 32 == 0.  Same outputs as k_dft<NBT, 0> up to FP32 rounding, with 2 (T/8) + 1 MFMAs per
+// 16 bins and 16 pixels instead of T/2.
 // Persistent like k_dft.  The second launch bound (resident blocks per CU) makes the compiler
 // keep the accumulators in ordinary registers instead of splitting them off into accumulation
-// registers: up to two tiles per wave fit in 136 registers, three waves per SIMD (three tiles
-// take two blocks per CU, 256 registers, so that they do not spill).
+// registers: up to two tiles fit in three waves per SIMD, three and four tiles in two.
 constexpr int fold_min_blocks(int nbf) { return nbf <= 2 ? 3 : 2; }
-template <int NBF, int SPLIT = 1>
+template <int NBF>
 __global__ __launch_bounds__(256, fold_min_blocks(NBF)) void k_dft_fold(DftArgs a) {
   extern __shared__ float4 lds_dyn[];
   float *lf = reinterpret_cast<float *>(lds_dyn);
-  stage_fold_matrix_rt(lf, reinterpret_cast<const float *>(a.wm_hi), a.r.T, NBF * SPLIT, a.nbc, threadIdx.x, 256);
+  stage_fold_matrix_rt(lf, reinterpret_cast<const float *>(a.wm_hi), a.r.T, NBF, a.nbc, threadIdx.x, 256);
   __syncthreads();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int part = SPLIT > 1 ? wave % SPLIT : 0;  // bin tiles part * NBF .. of this wave
-  constexpr int TPB = 4 / SPLIT;                   // pixel tiles per block iteration
   const long long tiles = (a.r.npix + 63) / 64;
   const bool lens = a.lx != nullptr;
-  for (long long t0 = (long long)blockIdx.x * TPB; t0 < tiles; t0 += (long long)gridDim.x * TPB) {
+  for (long long t0 = (long long)blockIdx.x * 4; t0 < tiles; t0 += (long long)gridDim.x * 4) {
     int lane;  // opaque per tile, as in k_dft
     asm volatile("v_mov_b32 %0, %1" : "=v"(lane) : "v"(threadIdx.x & 63));
     // tiles past the end load clamped pixels (dft_px) and store nothing
-    const long long p0w = (t0 + wave / SPLIT) * 64;
+    const long long p0w = (t0 + wave) * 64;
     f4 ac[NBF][4], as[NBF][4];
-    dft_fold_tile<NBF>(a.r, lf, p0w, lane, ac, as, part * NBF, NBF * SPLIT);
+    dft_fold_tile<NBF>(a.r, lf, p0w, lane, ac, as);
 
     const long long p0 = p0w + 4 * (lane & 15);
     if (p0 >= a.r.npix) continue;
@@ -180,7 +177,7 @@ __global__ __launch_bounds__(256, fold_min_blocks(NBF)) void k_dft_fold(DftArgs 
     for (int f = 0; f < NBF; ++f) {
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
-        const int bin = 16 * (part * NBF + f) + 4 * (lane >> 4) + rr;
+        const int bin = 16 * f + 4 * (lane >> 4) + rr;
         if (bin >= a.nbc) continue;
         float2 v[4];
 #pragma unroll
@@ -395,12 +392,15 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
 
   // producers: the band spectra of row i for their 128 pixels
   auto dft_row = [&](int i, int lane) {
-    const int j = i + 1 < nr ? i + 1 : i;  // the last row refills from itself (loads unused)
+    // the last row's refills (never used) all read the first pixels of the window's frames:
+    // every wave of every CU loads the same few KB, which stay in L2 instead of costing HBM reads
+    const bool next = i + 1 < nr;
+    const int j = next ? i + 1 : i;
+    const long long pn = next ? px0(row_y(j)) : 0;
     if constexpr (FOLD)
-      stream.row(ring_of(i), ring_of(j), wmat, px0(row_y(i)), px0(row_y(j)), lane, fre, fim);
+      stream.row(ring_of(i), ring_of(j), wmat, px0(row_y(i)), pn, lane, fre, fim);
     else
-      stream.row(ring_of(i), ring_of(j), reinterpret_cast<const float *>(wmat), px0(row_y(i)), px0(row_y(j)),
-                 lane, acc);
+      stream.row(ring_of(i), ring_of(j), reinterpret_cast<const float *>(wmat), px0(row_y(i)), pn, lane, acc);
   };
   // producers: MFMA registers of row i -> padded LDS rows (its window's lens for Fresnel)
   auto scatter = [&](int i, int lane) {
@@ -583,12 +583,11 @@ template <int NBT> static int launch_plain_mode(const DftArgs &a, int mode, hipS
   return -2;
 }
 
-template <int NBF, int SPLIT = 1> static int launch_fold(const DftArgs &a, hipStream_t st) {
-  const long long waves = (a.r.npix + 63) / 64 * SPLIT;
-  const size_t lds = dft_fold_lds_bytes(NBF * SPLIT, a.r.T);
-  const unsigned grid =
-      persistent_grid(reinterpret_cast<const void *>(&k_dft_fold<NBF, SPLIT>), lds, (waves + 3) / 4);
-  hipLaunchKernelGGL((k_dft_fold<NBF, SPLIT>), dim3(grid), dim3(256), lds, st, a);
+template <int NBF> static int launch_fold(const DftArgs &a, hipStream_t st) {
+  const long long waves = (a.r.npix + 63) / 64;
+  const size_t lds = dft_fold_lds_bytes(NBF, a.r.T);
+  const unsigned grid = persistent_grid(reinterpret_cast<const void *>(&k_dft_fold<NBF>), lds, (waves + 3) / 4);
+  hipLaunchKernelGGL((k_dft_fold<NBF>), dim3(grid), dim3(256), lds, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -600,7 +599,7 @@ int launch_time_dft(const DftLaunch &L, hipStream_t st) {
       case 1: return launch_fold<1>(a, st);
       case 2: return launch_fold<2>(a, st);
       case 3: return launch_fold<3>(a, st);
-      case 4: return launch_fold<2, 2>(a, st);
+      case 4: return launch_fold<4>(a, st);
     }
   }
   switch ((2 * L.nbc + 15) / 16) {

@@ -304,7 +304,7 @@ template <int NBT, int TPW, int T, int K> struct DftStream {
   }
   // D = Wm * X for the row at p0w of window r (operand matrix lw with pitch T + 4 floats);
   // the ring refills from this row, then from the row at p0w_next of window r_next.  On the
-  // last row the caller passes the row itself as the next one: those refills are never used,
+  // last row the caller passes any valid row as the next one: those refills are never used,
   // and issuing them anyway keeps every load out of branches (a conditional load makes the
   // steps after it wait for all loads in flight instead of their own).
   __device__ __forceinline__ void row(const DftRing &r, const DftRing &r_next, const float *lw, long long p0w,
@@ -436,14 +436,12 @@ template <int TPW, int T, int K> struct DftStreamFold {
 
 // Runtime-T form of the folded FP32 time DFT (see DftStreamFold) for NBF 16-bin tiles and one
 // 64-pixel tile per wave (stand-alone time DFT): steps of four frame pairs in batches of K,
-// T % (8 K) == 0.  Operand matrix from stage_fold_matrix_rt with pitch T/2 + 4.
-// f0, nt: the wave computes tiles f0 .. f0 + NBF - 1 of an operand matrix staged with nt tiles
-// (several waves may share one pixel tile and split its bins).
+// T % (8 K) == 0.  Operand matrix from stage_fold_matrix_rt (NBF tiles) with pitch T/2 + 4.
 template <int NBF, int K = 4>
 __device__ __forceinline__ void dft_fold_tile(const DftRing &r, const float *lds, long long p0w, int lane,
-                                              f4 (&ac)[NBF][4], f4 (&as)[NBF][4], int f0 = 0, int nt = NBF) {
+                                              f4 (&ac)[NBF][4], f4 (&as)[NBF][4]) {
   const int T = r.T, steps = T / 8, PC = T / 2 + 4;
-  const float *lc = lds + 16 * f0 * PC, *ls = lds + 16 * (nt + f0) * PC, *lx = lds + 32 * nt * PC + 64 * f0;
+  const float *lc = lds, *ls = lds + 16 * NBF * PC, *lx = lds + 32 * NBF * PC;
   const int g = lane >> 4, rw = lane & 15;
   const long long p0 = dft_px(r, p0w, lane, 0);
 #pragma unroll
