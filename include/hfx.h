@@ -117,7 +117,9 @@ hfx_status hfx_plan_get_info(const hfx_plan *p, hfx_plan_info *info);
 hfx_status hfx_plan_update_z(hfx_plan *p, double z_m, hfx_stream_t stream);
 /* One window: frames first_frame .. first_frame+T-1 (mod ring_frames) of a device ring of
  * ring_frames frames of H*W uint16 each.  `out` is a device buffer of n_outputs*OH*OW
- * floats: M0, M1, M2 [, M0 low band, M0 high band]. */
+ * floats: M0, M1, M2 [, M0 low band, M0 high band].  Frame and pixel indices are 32-bit in
+ * the kernels: rings of more than 2^31 - 1 frames (and, at hfx_plan_create, frames of more
+ * than 2^31 - 1 pixels) return HFX_E_UNSUPPORTED. */
 hfx_status hfx_moments(hfx_plan *p, const uint16_t *ring, int64_t ring_frames,
                        int64_t first_frame, float *out, hfx_stream_t stream);
 /* n_windows consecutive windows starting at first_frame, advancing by the plan stride;
