@@ -14,9 +14,10 @@
 // workgroup-wide Stockham.  Within a wave, LDS instructions complete in issue order, so
 // a write followed by a read of another lane's element needs only a compiler barrier.
 //
-// The scratch is padded by one element per 16 (pad(i) = i + i/16), which makes the
-// stride-E writes of pass 1 and the stride-NS accesses of the later passes free of bank
-// conflicts.  Pass twiddles come from an FP64-generated table tw[m] = exp(-2 pi i m/N);
+// The scratch is padded by one element per 16 (pad(i) = i + i/16; one per 32 at E = 32), which
+// makes the stride-E writes of pass 1 and the stride-NS accesses of the later passes free of
+// bank conflicts (at E = 32 the stride of 34 elements that one per 16 gives would put lanes l
+// and l + 16 on the same banks in pass 1; 33 spreads 32 lanes of 8-byte writes over all 64).  Pass twiddles come from an FP64-generated table tw[m] = exp(-2 pi i m/N);
 // the small in-register DFTs use the exact constants cos/sin(2 pi m/32) and (2 pi m/12).
 #pragma once
 
@@ -155,13 +156,14 @@ __host__ __device__ constexpr int wave_pass_radix(int rem, int E) {
 template <int N> struct WaveFft {
   static_assert(N % 64 == 0, "wave FFT length must be a multiple of 64");
   static constexpr int E = N / 64;
-  static constexpr int SCR = N + N / 16;  // padded scratch elements per transform
+  static constexpr int PSH = E >= 32 ? 5 : 4;  // one pad element per 2^PSH
+  static constexpr int SCR = N + (N >> PSH);    // padded scratch elements per transform
   // Pass twiddles are stored pass-major: the pass with stride NS and radix R owns
   // (R-1)*NS entries W_{NS*R}^{r*k} at [r-1][k], so the lanes of a wave (consecutive k)
   // read consecutive entries.  Over all passes this is N - E entries.
   static constexpr int TW = N - E;
 
-  __device__ __forceinline__ static int pad(int i) { return i + (i >> 4); }
+  __device__ __forceinline__ static int pad(int i) { return i + (i >> PSH); }
 
   // offset of the table of the pass that starts at stride NS
   static constexpr int tw_offset(int ns) {
