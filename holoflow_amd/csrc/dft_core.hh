@@ -98,8 +98,9 @@ __host__ __device__ constexpr size_t dft_matrix_lds_bytes(int nbt, int mode, int
 template <int MODE> constexpr int dft_pre_steps() { return 8; }
 
 __device__ __forceinline__ int ring_frame(const DftRing &r, int n) {
-  const int f = (int)r.first + n, R = (int)r.ring_frames;
-  return f >= R ? f - R : f;  // n < T <= ring_frames
+  // unsigned: first + n may pass 2^31 - 1 for a ring of nearly 2^31 frames
+  const unsigned f = (unsigned)r.first + (unsigned)n, R = (unsigned)r.ring_frames;
+  return (int)(f >= R ? f - R : f);  // n < T <= ring_frames
 }
 
 // The wave owns TPW adjacent 64-pixel tiles starting at p0w (a multiple of 4); tile i
