@@ -13,7 +13,9 @@ over the timed steps.
 Also reported on the same line: the mixed-precision MFMA time-DFT variants (throughput and
 relative L2 of every moment image against the FP64 oracle), per-window GPU latency
 percentiles, the roofline of the dominant kernel (the time DFT fused with the row FFT) and
-of the second kernel (column FFT + moments), the copy ceiling, and the CPU baseline
+of the second kernel (column FFT + moments), the practical HBM ceilings of the same GPU
+(the library's bandwidth probe reading, writing and copying 4 GiB), the 2048x2048 T=128
+stream, the serving legs (mixed streams and a same-shape camera rig), and the CPU baseline
 (the oracle's FP32 torch-CPU/MKL restatement on a bounded sample of windows).
 """
 from __future__ import annotations
