@@ -145,16 +145,20 @@ def test_folded_front_vs_oracles(gpu, W, T, kb, ke, prop, both):
     _check(p, _frames(64, W, F), F - T // 4)
 
 
-@pytest.mark.parametrize("T,kb,ke,prop,both", [(64, 1, 30, "asm", False),    # 29 bins: two tiles
-                                               (32, 0, 16, "fresnel", True),  # 17 bins with DC: two
-                                               (128, 10, 50, "fresnel", False)])  # 40 bins: three
-def test_folded_standalone_dft_vs_oracles(gpu, T, kb, ke, prop, both):
+@pytest.mark.parametrize("H,W,T,kb,ke,prop,both", [(48, 64, 64, 1, 30, "asm", False),    # 29 bins: two tiles
+                                                   (48, 64, 32, 0, 16, "fresnel", True),  # 17 bins with DC
+                                                   (48, 64, 128, 10, 50, "fresnel", False),  # 40: three
+                                                   (4, 24, 128, 1, 60, "asm", False),  # 59: four, ragged
+                                                   (4, 24, 32, 3, 9, "fresnel", False)])  # 6: interleaved
+def test_folded_standalone_dft_vs_oracles(gpu, H, W, T, kb, ke, prop, both):
     """The stand-alone FP32 time DFT (widths without the fused kernel) folds the real-input
-    symmetry over several 16-bin tiles."""
-    p = O.PathParams(48, 64, T, T // 4, kb, ke, prop, 852e-9, 12e-6, 12e-6, 0.35, fftshift=True,
+    symmetry over several 16-bin tiles (up to 8 bins: the interleaved cos/-sin contraction).
+    The persistent kernels walk 64-pixel tiles; at 4 x 24 = 96 pixels the second tile is
+    ragged (its lanes past the end load clamped pixels and store nothing)."""
+    p = O.PathParams(H, W, T, T // 4, kb, ke, prop, 852e-9, 12e-6, 12e-6, 0.35, fftshift=True,
                      both_sidebands=both)
     F = T + T // 2
-    _check(p, _frames(48, 64, F), F - T // 4)
+    _check(p, _frames(H, W, F), F - T // 4)
 
 
 def test_fresnel_band_split_1024_rows(gpu):
