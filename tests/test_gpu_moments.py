@@ -170,7 +170,9 @@ def test_fresnel_band_split_1024_rows(gpu):
 
 
 @pytest.mark.parametrize("H,W,T,kb,ke,split", [(256, 512, 64, 5, 30, 15),    # 25 bins: stand-alone DFT
-                                                 (128, 1024, 32, 3, 12, 7)])    # 9 bins, 1024-point rows
+                                                 (128, 1024, 32, 3, 12, 7),     # 9 bins, 1024-point rows
+                                                 (2, 2048, 32, 3, 12, 7),       # 2048-point rows, 2 rows
+                                                 (64, 2048, 64, 5, 30, 0)])     # 2048-point rows, no split
 def test_asm_band_split_from_the_row_pass(gpu, H, W, T, kb, ke, split):
     """Angular spectrum with the band split: the moments come out of the inverse row pass,
     which runs the planes below the split and those above it as two loops (M0 low stored at
@@ -178,7 +180,7 @@ def test_asm_band_split_from_the_row_pass(gpu, H, W, T, kb, ke, split):
     16-bin tiles) and the forward row pass reads its natural-layout planes."""
     p = O.PathParams(H, W, T, T // 4, kb, ke, "asm", 852e-9, 12e-6, 12e-6, 0.35, fftshift=True, split_bin=split)
     out, _ = _check(p, _frames(H, W, T + T // 2, radius=0.4 * H), T // 4)
-    assert out.shape == (5, H, W)
+    assert out.shape == (5 if split else 3, H, W)
 
 
 def test_batch_matches_single_windows(gpu):
