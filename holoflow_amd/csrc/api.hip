@@ -272,16 +272,16 @@ hfx_status hfx_plan_create(const hfx_plan_desc *dp, hfx_stream_t stream, hfx_pla
   DevArena &A = p->arena;
   p->filter2d = d.prop == HFX_PROP_FRESNEL && d.filter_enabled;
   p->fused_row = !p->filter2d && p->OH == p->H && p->OW == p->W && hfx::dft_row_supported(p->W, p->nbc, d.window);
-  // natural-layout planes: only the stand-alone time DFT needs them
-  HFX_TRY(A.alloc(&p->planes_nat, p->fused_row ? 1 : (size_t)p->n_planes * p->npix));
   if (d.batch_windows > 1 && p->fused_row &&
       hfx::dft_front_pipelined(p->W, p->nbc, d.window, p->n_planes, d.precision))
     p->nwin = std::min<int>(d.batch_windows, hfx::HFX_MAX_WIN);
-  // batch lanes: a second stream (and plane workspace) lets one window's front overlap the
-  // previous window's back (measured +15 % at 1024^2, T = 64, in FP32 and the 16-bit modes);
-  // the Flatfield's scratch images are shared, so it keeps one lane.
+  // batch lanes: a second stream (and plane workspaces) lets one window's kernels overlap the
+  // neighbouring window's (measured +15 % at 1024^2, T = 64, in FP32 and the 16-bit modes);
+  // the Flatfield's scratch images and Filter2D's tables are shared, so those keep one lane.
   if (d.batch_lanes < 0 || d.batch_lanes > 2) return fail(HFX_E_INVALID, "batch_lanes must be 0, 1 or 2");
-  p->lanes = d.batch_lanes != 1 && p->fused_row && !d.flatfield_enabled ? 2 : 1;
+  p->lanes = d.batch_lanes != 1 && !p->filter2d && !d.flatfield_enabled ? 2 : 1;
+  // natural-layout planes, one set per lane: only the stand-alone time DFT needs them
+  HFX_TRY(A.alloc(&p->planes_nat, p->fused_row ? 1 : (size_t)p->lanes * p->n_planes * p->npix));
   if (p->lanes == 2) {
     HFX_TRY_HIP(hipStreamCreateWithFlags(&p->aux, hipStreamNonBlocking));
     HFX_TRY_HIP(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
@@ -421,17 +421,21 @@ static hfx::PassArgs base_pass(int H, int W, int planes, int blk_c) {
 
 // Front of the schedule: the window's band planes, row-transformed, in the plan's blocked
 // workspace (one fused kernel when the frame width allows, else time DFT + row pass).
+// planes: the blocked workspace (default the plan's first); lane: the natural-layout planes'
+// set of the stand-alone time DFT
 static hfx_status window_front(hfx_plan *p, const uint16_t *ring, int64_t ring_frames, int64_t first,
-                               hipStream_t st, float2 *planes = nullptr) {
+                               hipStream_t st, float2 *planes = nullptr, int lane = 0) {
+  if (!planes) planes = p->planes_blk;
   if (p->fused_row) {
-    hfx::DftLaunch L = dft_launch(p, ring, ring_frames, first, planes ? planes : p->planes_blk);
+    hfx::DftLaunch L = dft_launch(p, ring, ring_frames, first, planes);
     if (hfx::launch_time_dft_row(L, p->OH, p->blk_c, p->n_planes, p->tw_w, st) != 0)
       return fail(HFX_E_HIP, "fused time-DFT/row-FFT launch failed");
     return HFX_OK;
   }
-  HFX_TRY(run_dft(p, ring, ring_frames, first, p->planes_nat, st));
-  // Filter2D in place on the natural planes, through the blocked workspace
-  if (p->filter2d) HFX_TRY(spectral_run(p->f2d, p->planes_nat, false, p->planes_nat, false, st));
+  float2 *nat = p->planes_nat + (long long)lane * p->n_planes * p->npix;
+  HFX_TRY(run_dft(p, ring, ring_frames, first, nat, st));
+  // Filter2D in place on the natural planes, through the blocked workspace (one lane)
+  if (p->filter2d) HFX_TRY(spectral_run(p->f2d, nat, false, nat, false, st));
   // row pass: natural planes (lens already applied for Fresnel, or here after Filter2D)
   // -> blocked spectrum rows
   hfx::PassArgs r = base_pass(p->OH, p->OW, p->n_planes, p->blk_c);
@@ -439,13 +443,13 @@ static hfx_status window_front(hfx_plan *p, const uint16_t *ring, int64_t ring_f
     r.pre_lx = p->lx;
     r.pre_ly = p->ly;
   }
-  r.in = p->planes_nat;
+  r.in = nat;
   r.in_mode = hfx::IN_NAT_C;
   r.in_h = p->H;
   r.in_w = p->W;
   r.off_y = (p->OH - p->H) / 2;
   r.off_x = (p->OW - p->W) / 2;
-  r.out = p->planes_blk;
+  r.out = planes;
   r.out_mode = hfx::OUT_BLK;
   r.tw = p->tw_w;
   if (hfx::launch_row_pass(r, st) != 0) return fail(HFX_E_HIP, "row pass launch failed");
@@ -652,7 +656,7 @@ hfx_status hfx_moments_batch(hfx_plan *p, const uint16_t *ring, int64_t ring_fra
       }
       for (int j = 0; j < k && rc == HFX_OK; ++j) rc = window_back(p, out + (w + j) * out_stride, ls, ws + j * pstride);
     } else {
-      rc = window_front(p, ring, ring_frames, frame_of(w), ls, ws);
+      rc = window_front(p, ring, ring_frames, frame_of(w), ls, ws, lane);
       if (rc == HFX_OK) rc = window_back(p, out + w * out_stride, ls, ws);
     }
   }

@@ -84,12 +84,12 @@ typedef struct hfx_plan_desc {
                                 the cost of one plane workspace each; FP32 Fresnel/ASM
                                 without padding or Filter2D, else 1) */
   int32_t batch_lanes;       /* hfx_moments_batch: HIP streams the windows alternate over, so
-                                that one window's front kernel overlaps the previous window's
-                                back kernel (0 or 2 = two wherever the time DFT is fused with
-                                the row FFT and no Flatfield runs, else one; 1 = the caller's
-                                stream only).  Each lane holds its own plane workspace; the
-                                second lane runs on a stream the plan owns and joins the
-                                caller's stream before the call returns. */
+                                that one window's kernels overlap the neighbouring window's
+                                (0 or 2 = two unless a Flatfield or a Fresnel Filter2D stage
+                                runs, else one; 1 = the caller's stream only).  Each lane
+                                holds its own plane workspaces; the second lane runs on a
+                                stream the plan owns and joins the caller's stream before the
+                                call returns. */
 } hfx_plan_desc;
 
 typedef struct hfx_plan_info {

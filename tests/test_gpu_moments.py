@@ -217,12 +217,15 @@ def test_multi_window_front_matches_single_windows(gpu, prop, W, nwin):
 
 
 @pytest.mark.parametrize("prop,W,nwin,prec", [("fresnel", 1024, 1, "fp32"), ("asm", 512, 2, "fp32"),
-                                               ("fresnel", 512, 1, "fp16_split")])
+                                               ("fresnel", 512, 1, "fp16_split"),
+                                               ("asm", 2048, 1, "fp32"), ("fresnel", 2048, 1, "fp32")])
 def test_two_lanes_match_one_lane(gpu, prop, W, nwin, prec):
     """batch_lanes = 2: window groups alternate between the caller's stream and the plan's own
     stream (forked from and joined into the caller's).  Every window must equal the one-lane
     result bit for bit once the caller's stream is synchronised, on a non-default caller stream
-    too; a Flatfield plan keeps one lane (shared scratch images)."""
+    too; at 2048 wide the time DFT runs stand-alone, each lane with its own natural-layout
+    planes.  A Flatfield plan (shared scratch images) and a Filter2D plan (shared tables) keep
+    one lane."""
     from holoflow_amd import hfx
 
     T, S, F = 64, 16, 200
@@ -250,7 +253,7 @@ def test_two_lanes_match_one_lane(gpu, prop, W, nwin, prec):
     assert torch.equal(two.moments_batch(ring, first, 1)[0], ref[0])
     assert torch.equal(two.moments_batch(ring, first, nwin + 1), ref[: nwin + 1])
 
-    # library default: two lanes wherever the front is fused
+    # library default: two lanes
     assert hfx.MomentPlan(_cfg(p, prec)).batch_lanes == 2
     cfg = _cfg(p, prec)
     cfg.flatfield = (3.0, 3.0)
@@ -258,6 +261,10 @@ def test_two_lanes_match_one_lane(gpu, prop, W, nwin, prec):
     assert ff_default.batch_lanes == 1
     cfg.batch_lanes = 1
     assert ff_default.workspace_bytes == hfx.MomentPlan(cfg).workspace_bytes
+    if prop == "fresnel":
+        cfg = _cfg(p, prec)
+        cfg.filt = (4, 20, 2, 4)
+        assert hfx.MomentPlan(cfg).batch_lanes == 1
 
 
 def test_two_lane_batch_in_hip_graph(gpu):
