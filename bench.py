@@ -15,8 +15,9 @@ relative L2 of every moment image against the FP64 oracle), per-window GPU laten
 percentiles, the roofline of the dominant kernel (the time DFT fused with the row FFT) and
 of the second kernel (column FFT + moments), the practical HBM ceilings of the same GPU
 (the library's bandwidth probe reading, writing and copying 4 GiB), the 2048x2048 T=128
-stream, the serving legs (mixed streams and a same-shape camera rig), and the CPU baseline
-(the oracle's FP32 torch-CPU/MKL restatement on a bounded sample of windows).
+stream, the same path fed from pinned host memory (PCIe-inclusive, `host_ingest`), the
+serving legs (mixed streams and a same-shape camera rig), and the CPU baseline (the oracle's
+FP32 torch-CPU/MKL restatement on a bounded sample of windows).
 """
 from __future__ import annotations
 
@@ -119,6 +120,67 @@ def hbm_probe_leg(torch, stream, nbytes=4 << 30):
     del a, b
     torch.cuda.empty_cache()
     return res
+
+
+def host_ingest_leg(torch, plan, ring, n_windows, queue=8, host_frames=512):
+    """The headline workload with its frames handed over in host memory instead of resident
+    in HBM (PCIe-inclusive; never the headline value).  The stream sits in pinned host memory
+    (host_frames frames, replayed cyclically); a device ring of T + queue*S frames receives
+    each window's S new frames on a copy stream while earlier windows compute on another
+    (each copy waits for the window that last read the slots it overwrites).  Also the pinned
+    host-to-device copy rate alone over the same frames."""
+    npix = H * W
+    src = ring[:host_frames].cpu().pin_memory()
+    R = T + queue * S
+    dev = torch.empty((R, H, W), dtype=torch.int16, device="cuda")
+    cs, st = torch.cuda.Stream(), torch.cuda.Stream()
+    out = plan.new_output(queue + 2)
+
+    def copy(f0, n):  # stream frames [f0, f0 + n) into their ring slots, on cs
+        f = f0
+        while f < f0 + n:
+            rs, hs = f % R, f % host_frames
+            k = min(f0 + n - f, R - rs, host_frames - hs)
+            dev[rs:rs + k].copy_(src[hs:hs + k], non_blocking=True)
+            f += k
+
+    def run(nw):
+        done = [torch.cuda.Event() for _ in range(nw)]
+        with torch.cuda.stream(cs):
+            copy(0, T - S)
+        for w in range(nw):
+            with torch.cuda.stream(cs):
+                if w >= queue:
+                    cs.wait_event(done[w - queue])  # frees the slots of frames < (w - queue) S + T
+                copy(w * S + T - S, S)
+            ready = torch.cuda.Event()
+            ready.record(cs)
+            st.wait_event(ready)
+            plan.moments(dev, (w * S) % R, out[w % (queue + 2)], stream=st)
+            done[w].record(st)
+        torch.cuda.synchronize()
+
+    run(2 * queue)  # warm-up
+    t0 = time.perf_counter()
+    run(n_windows)
+    wps = n_windows / (time.perf_counter() - t0)
+    # the pinned copy alone: host_frames frames to the device, twice
+    blk = torch.empty((host_frames, H, W), dtype=torch.int16, device="cuda")
+    blk.copy_(src, non_blocking=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        blk.copy_(src, non_blocking=True)
+    torch.cuda.synchronize()
+    h2d = 2 * host_frames * npix * 2 / (time.perf_counter() - t0) / 1e9
+    del src, dev, blk, out
+    torch.cuda.empty_cache()
+    return {"value": round(wps, 1), "unit": "moment frames/s",
+            "h2d_gbs_in_path": round(wps * S * npix * 2 / 1e9, 2), "h2d_gbs_copy_alone": round(h2d, 2),
+            "windows": n_windows,
+            "note": f"frames in pinned host memory ({host_frames} frames replayed), S new frames per "
+                    f"window copied into a {R}-frame device ring on a copy stream while earlier windows "
+                    f"compute (one hfx_moments per window); PCIe-inclusive, not the headline value"}
 
 
 def cfg4_leg(args, np, torch):
@@ -360,6 +422,7 @@ def main():
     ap.add_argument("--no-cfg4", action="store_true")
     ap.add_argument("--no-stages", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-host-ingest", action="store_true")
     ap.add_argument("--graph-seconds", type=float, default=3.0)
     ap.add_argument("--cfg4-frames", type=int, default=8192)
     ap.add_argument("--serving-streams", type=int, default=24)
@@ -497,6 +560,11 @@ def main():
                                              "source": "design measurement (fused fronts built without the time DFT)"}
 
     del out
+    # ---- the same path with the frames handed over in host memory (PCIe-inclusive)
+    host_ingest = None
+    if not args.no_host_ingest:
+        host_ingest = host_ingest_leg(torch, plan, ring, n_windows)
+        log(f"host ingest: {host_ingest}")
     # ---- optional in-path stages (Filter2D, Flatfield) at the workload's sizes
     stages = None
     if not args.no_stages:
@@ -581,6 +649,7 @@ def main():
                        "window_gpu_p99": round(float(np.percentile(lat, 99)), 4),
                        "window_period": round(window_period_ms, 4)},
         "mixed": mixed,
+        "host_ingest": host_ingest,
         "stages": stages,
         "graph": graph_res,
         "cfg4": big,
