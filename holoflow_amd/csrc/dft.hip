@@ -47,40 +47,55 @@ struct DftArgs {
   const float2 *lx_w[HFX_MAX_WIN], *ly_w[HFX_MAX_WIN];
 };
 
-// true for a computed bin that is the DC bin: bin 0, or bin T itself, the extra bin the
-// two-sided path computes when its band ends at T (its conjugate plane is X[0])
-__device__ __forceinline__ bool dc_bin(const DftArgs &a, int bin) {
-  const int k = a.kb + bin;
-  return k == 0 || k == a.r.T;
+// a wave-uniform value kept in a scalar register from here on (an opaque copy the compiler
+// cannot rematerialise by reloading the kernel argument)
+__device__ __forceinline__ int pin_sgpr(int v) {
+  asm volatile("" : "+s"(v));
+  return v;
 }
 
-// value of computed bin `bin` for 4 pixels, with the centring offset restored on the DC bin
-__device__ __forceinline__ void dft_bin_values(const DftArgs &a, const f4 (&acc)[4], int h, int bin,
+// The band scalars every bin store tests, pinned in scalar registers at the top of a kernel:
+// left to the compiler they are reloaded from the kernel arguments inside the stores'
+// branches, and each reload's wait (lgkmcnt) also waits for the LDS writes in flight.
+struct Band {
+  int nb, nbc, kb, both, T;
+};
+__device__ __forceinline__ Band pinned_band(const DftArgs &a) {
+  return {pin_sgpr(a.nb), pin_sgpr(a.nbc), pin_sgpr(a.kb), pin_sgpr(a.both), pin_sgpr(a.r.T)};
+}
+
+// Centring offset of computed bin `bin`: 2048 T on a DC bin, else 0.  The DC bin is bin 0, or
+// bin T itself, the extra bin the two-sided path computes when its band ends at T (its
+// conjugate plane is X[0]).  Added to every bin: a select instead of a branch around the
+// additions (adding +0 changes no other value, only the sign of a zero).
+__device__ __forceinline__ float dc_offset(int kb, int T, int bin) {
+  return kb + bin == 0 || kb + bin == T ? 2048.0f * (float)T : 0.0f;
+}
+
+// value of computed bin `bin` for 4 pixels, with the centring offset restored
+__device__ __forceinline__ void dft_bin_values(const Band &b, const f4 (&acc)[4], int h, int bin,
                                                float2 (&v)[4]) {
+  const float dc = dc_offset(b.kb, b.T, bin);
 #pragma unroll
-  for (int t = 0; t < 4; ++t) v[t] = make_float2(acc[t][2 * h], acc[t][2 * h + 1]);
-  if (dc_bin(a, bin)) {
-#pragma unroll
-    for (int t = 0; t < 4; ++t) v[t].x += 2048.0f * (float)a.r.T;
-  }
+  for (int t = 0; t < 4; ++t) v[t] = make_float2(acc[t][2 * h] + dc, acc[t][2 * h + 1]);
 }
 
 // ----------------------------------------------------------------------- stand-alone
 // Band bin `bin` of the 4 pixels from p0 into its natural-layout plane (and the conjugate
 // plane of the two-sided path), times the lens when there is one.
-__device__ __forceinline__ void dft_store_bin(const DftArgs &a, long long p0, int bin, const float2 (&v)[4],
-                                              const float2 (&L)[4], bool lens) {
+__device__ __forceinline__ void dft_store_bin(const DftArgs &a, const Band &b, long long p0, int bin,
+                                              const float2 (&v)[4], const float2 (&L)[4], bool lens) {
 #pragma unroll
   for (int side = 0; side < 2; ++side) {
     int q;
     if (side == 0) {
-      if (bin >= a.nb) continue;
+      if (bin >= b.nb) continue;
       q = bin;
     } else {
-      if (!a.both) continue;
-      const int j = a.nb - bin;
-      if (j < 0 || j >= a.nb) continue;
-      q = a.nb + j;
+      if (!b.both) continue;
+      const int j = b.nb - bin;
+      if (j < 0 || j >= b.nb) continue;
+      q = b.nb + j;
     }
     float2 o[4];
 #pragma unroll
@@ -101,6 +116,7 @@ __global__ __launch_bounds__(256) void k_dft(DftArgs a) {
   extern __shared__ float4 lds_dyn[];
   dft_stage_matrix<NBT, MODE>(lds_dyn, a.wm_hi, a.wm_lo, a.r.T, threadIdx.x, 256);
   __syncthreads();
+  const Band b = pinned_band(a);
   const long long tiles = (a.r.npix + 63) / 64;
   const bool lens = a.lx != nullptr;
   for (long long t0 = (long long)blockIdx.x * 4; t0 < tiles; t0 += (long long)gridDim.x * 4) {
@@ -130,10 +146,10 @@ __global__ __launch_bounds__(256) void k_dft(DftArgs a) {
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const int bin = bt * 8 + 2 * (lane >> 4) + h;
-        if (bin >= a.nbc) continue;
+        if (bin >= b.nbc) continue;
         float2 v[4];
-        dft_bin_values(a, acc[bt], h, bin, v);
-        dft_store_bin(a, p0, bin, v, L, lens);
+        dft_bin_values(b, acc[bt], h, bin, v);
+        dft_store_bin(a, b, p0, bin, v, L, lens);
       }
     }
   }
@@ -153,6 +169,7 @@ __global__ __launch_bounds__(256, fold_min_blocks(NBF)) void k_dft_fold(DftArgs 
   float *lf = reinterpret_cast<float *>(lds_dyn);
   stage_fold_matrix_rt(lf, reinterpret_cast<const float *>(a.wm_hi), a.r.T, NBF, a.nbc, threadIdx.x, 256);
   __syncthreads();
+  const Band b = pinned_band(a);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long long tiles = (a.r.npix + 63) / 64;
   const bool lens = a.lx != nullptr;
@@ -178,15 +195,12 @@ __global__ __launch_bounds__(256, fold_min_blocks(NBF)) void k_dft_fold(DftArgs 
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
         const int bin = 16 * f + 4 * (lane >> 4) + rr;
-        if (bin >= a.nbc) continue;
+        if (bin >= b.nbc) continue;
+        const float dc = dc_offset(b.kb, b.T, bin);
         float2 v[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) v[t] = make_float2(ac[f][t][rr], as[f][t][rr]);
-        if (dc_bin(a, bin)) {
-#pragma unroll
-          for (int t = 0; t < 4; ++t) v[t].x += 2048.0f * (float)a.r.T;
-        }
-        dft_store_bin(a, p0, bin, v, L, lens);
+        for (int t = 0; t < 4; ++t) v[t] = make_float2(ac[f][t][rr] + dc, as[f][t][rr]);
+        dft_store_bin(a, b, p0, bin, v, L, lens);
       }
     }
   }
@@ -223,6 +237,7 @@ __global__ __launch_bounds__(row_threads<W>()) void k_dft_row(DftArgs a) {
   F::build_twiddles(twl, a.tw, tid, NT);
   __syncthreads();
 
+  const Band b = pinned_band(a);
   const long long plane_elems = (long long)a.H * W;
   const bool lens = a.lx != nullptr;
   const int G = gridDim.x;
@@ -257,20 +272,20 @@ __global__ __launch_bounds__(row_threads<W>()) void k_dft_row(DftArgs a) {
 #pragma unroll
           for (int h = 0; h < 2; ++h) {
             const int bin = bt * 8 + 2 * (lane >> 4) + h;
-            if (bin >= a.nbc) continue;
+            if (bin >= b.nbc) continue;
             float2 v[4];
-            dft_bin_values(a, acc[i][bt], h, bin, v);
+            dft_bin_values(b, acc[i][bt], h, bin, v);
 #pragma unroll
             for (int side = 0; side < 2; ++side) {
               int q;
               if (side == 0) {
-                if (bin >= a.nb) continue;
+                if (bin >= b.nb) continue;
                 q = bin;
               } else {
-                if (!a.both) continue;
-                const int j = a.nb - bin;
-                if (j < 0 || j >= a.nb) continue;
-                q = a.nb + j;
+                if (!b.both) continue;
+                const int j = b.nb - bin;
+                if (j < 0 || j >= b.nb) continue;
+                q = b.nb + j;
               }
               if (q < q0 || q >= q0 + GB) continue;
               float2 *dst = scr + (q - q0) * F::SCR;
@@ -402,6 +417,7 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
     else
       stream.row(ring_of(i), ring_of(j), reinterpret_cast<const float *>(wmat), px0(row_y(i)), pn, lane, acc);
   };
+  const Band b = pinned_band(a);
   // producers: MFMA registers of row i -> padded LDS rows (its window's lens for Fresnel)
   auto scatter = [&](int i, int lane) {
     const int y = row_y(i);
@@ -413,13 +429,13 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
       for (int side = 0; side < 2; ++side) {
         int q;
         if (side == 0) {
-          if (bin >= a.nb) continue;
+          if (bin >= b.nb) continue;
           q = bin;
         } else {
-          if (!a.both) continue;
-          const int j = a.nb - bin;
-          if (j < 0 || j >= a.nb) continue;
-          q = a.nb + j;
+          if (!b.both) continue;
+          const int j = b.nb - bin;
+          if (j < 0 || j >= b.nb) continue;
+          q = b.nb + j;
         }
         float2 *dst = scr + q * F::SCR;
 #pragma unroll
@@ -442,15 +458,14 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
       if constexpr (FOLD) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
+          // bins past the computed ones fail put's own tests (nbc = nb, or nb + 1 on the
+          // two-sided path, whose conjugate planes use bins 1..nb)
           const int bin = 4 * (lane >> 4) + r;
-          if (bin >= a.nbc) continue;
+          // the centring offset restored on the DC bin: a select, not a branch
+          const float dc = dc_offset(b.kb, T, bin);
           float2 v[4];
 #pragma unroll
-          for (int t = 0; t < 4; ++t) v[t] = make_float2(fre[i][t][r], fim[i][t][r]);
-          if (dc_bin(a, bin)) {  // the centring offset restored on the DC bin
-#pragma unroll
-            for (int t = 0; t < 4; ++t) v[t].x += 2048.0f * (float)a.r.T;
-          }
+          for (int t = 0; t < 4; ++t) v[t] = make_float2(fre[i][t][r] + dc, fim[i][t][r]);
           put(x0, bin, v, L);
         }
       } else {
@@ -459,9 +474,9 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
 #pragma unroll
           for (int h = 0; h < 2; ++h) {
             const int bin = bt * 8 + 2 * (lane >> 4) + h;
-            if (bin >= a.nbc) continue;
+            if (bin >= b.nbc) continue;
             float2 v[4];
-            dft_bin_values(a, acc[i][bt], h, bin, v);
+            dft_bin_values(b, acc[i][bt], h, bin, v);
             put(x0, bin, v, L);
           }
         }
