@@ -209,7 +209,7 @@ def cfg4_leg(args, np, torch):
         plan = hfx.MomentPlan(hfx.MomentConfig(H4, W4, p.window, p.stride, p.bin_begin, p.bin_end, "asm",
                                                p.lambda_m, p.dx_m, p.dy_m, p.z_m, fs_hz=10e3,
                                                weight_fs_hz=p.weight_fs_hz, precision=prec,
-                                               split_bin=p.split_bin))
+                                               split_bin=p.split_bin, batch_lanes=args.batch_lanes))
         if out is None:
             out = plan.new_output(n_win)
         dt = time_batch(plan, ring, n_win, out, 2, 1, torch)
