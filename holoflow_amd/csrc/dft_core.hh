@@ -140,27 +140,6 @@ __device__ __forceinline__ void unpack4(uint2 raw, float (&x)[4]) {
   x[3] = (float)(raw.y >> 16) - 2048.0f;
 }
 
-// Folded operands of four pixels from their samples at frames p (a) and T - p (b):
-// s = (x_p - 2048) + (x_{T-p} - 2048) and d = x_p - x_{T-p}, formed exactly in 16-bit integer
-// arithmetic (s in [-4096, 4094], d in [-4095, 4095] for 12-bit samples) with packed adds, two
-// pixels per instruction, then one conversion per value: 14 instructions for the four pairs
-// where unpacking each sample to FP32 first takes 24.  The same values as the FP32 form.
-__device__ __forceinline__ void fold4(uint2 a, uint2 b, float (&s)[4], float (&d)[4]) {
-  typedef unsigned short us2 __attribute__((ext_vector_type(2)));
-  typedef short ss2 __attribute__((ext_vector_type(2)));
-  const unsigned aw[2] = {a.x, a.y}, bw[2] = {b.x, b.y};
-#pragma unroll
-  for (int w = 0; w < 2; ++w) {
-    const us2 ua = __builtin_bit_cast(us2, aw[w]), ub = __builtin_bit_cast(us2, bw[w]);
-    const ss2 sw = __builtin_bit_cast(ss2, (us2)(ua + ub - (us2){4096, 4096}));
-    const ss2 dw = __builtin_bit_cast(ss2, ua) - __builtin_bit_cast(ss2, ub);
-    s[2 * w] = (float)sw.x;
-    s[2 * w + 1] = (float)sw.y;
-    d[2 * w] = (float)dw.x;
-    d[2 * w + 1] = (float)dw.y;
-  }
-}
-
 // MFMA loop over the window, starting from the prefetched first 32 frames `pre` and
 // loading the rest one batch ahead of its use.
 template <int NBT, int MODE, int TPW, int PRE = dft_pre_steps<MODE>(), int KB = 8 / TPW>
@@ -421,7 +400,16 @@ template <int TPW, int T, int K> struct DftStreamFold {
       const int slot = s % K;
       float sa[TPW][4], sb[TPW][4];
 #pragma unroll
-      for (int i = 0; i < TPW; ++i) fold4(ring[slot][i][0], ring[slot][i][1], sa[i], sb[i]);
+      for (int i = 0; i < TPW; ++i) {
+        float x1[4], x2[4];
+        unpack4(ring[slot][i][0], x1);
+        unpack4(ring[slot][i][1], x2);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          sa[i][t] = x1[t] + x2[t];  // exact: centred 12-bit samples
+          sb[i][t] = x1[t] - x2[t];
+        }
+      }
       if (s + K < STEPS)
         load(r, p0w, lane, s + K, slot);
       else  // unconditional: see DftStream::row
@@ -477,8 +465,14 @@ __device__ __forceinline__ void dft_fold_tile(const DftRing &r, const float *lds
   // the last batch is peeled so that no load sits behind a branch (a conditional load makes
   // every step wait for all loads in flight instead of its own)
   auto step = [&](int st, int k, auto load_next) {
-    float sa[4], sb[4];
-    fold4(ring[k][0], ring[k][1], sa, sb);
+    float x1[4], x2[4], sa[4], sb[4];
+    unpack4(ring[k][0], x1);
+    unpack4(ring[k][1], x2);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      sa[t] = x1[t] + x2[t];
+      sb[t] = x1[t] - x2[t];
+    }
     if constexpr (decltype(load_next)::value) {
       load(st + K, k);
       // issue the refill here: left to itself the scheduler sinks all refills of a batch to

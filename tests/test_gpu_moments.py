@@ -161,6 +161,21 @@ def test_folded_standalone_dft_vs_oracles(gpu, H, W, T, kb, ke, prop, both):
     _check(p, _frames(H, W, F), F - T // 4)
 
 
+@pytest.mark.parametrize("H,W,kb,ke,prop", [(16, 512, 11, 26, "fresnel"),  # fused front, one tile
+                                              (24, 64, 1, 30, "asm")])       # stand-alone, two tiles
+def test_full_16bit_samples_fp32(gpu, H, W, kb, ke, prop):
+    """The FP32 path takes any uint16 sample, not only 12-bit ones: the folded time DFT forms
+    x_p + x_{T-p} and x_p - x_{T-p} exactly for the whole 16-bit range (uniform samples over
+    [0, 65535], against both oracles)."""
+    T = 64
+    p = O.PathParams(H, W, T, T // 4, kb, ke, prop, 852e-9, 12e-6, 12e-6, 0.4, fftshift=True)
+    rng = np.random.default_rng(16)
+    host = rng.integers(0, 65536, size=(T + 8, H, W), dtype=np.uint16)
+    assert host.max() > 60000
+    ring = torch.from_numpy(host.view(np.int16)).cuda()
+    _check(p, ring, 3)
+
+
 def test_fresnel_band_split_1024_rows(gpu):
     """Band-split moments (five sums) out of the 1024-row column pass, against both oracles."""
     p = O.PathParams(1024, 256, 32, 8, 4, 12, "fresnel", 852e-9, 12e-6, 12e-6, 0.45, fftshift=True,
