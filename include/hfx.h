@@ -50,7 +50,7 @@ enum hfx_prop { HFX_PROP_FRESNEL = 0, HFX_PROP_ANGULAR_SPECTRUM = 1 };
 enum hfx_precision {
   HFX_PREC_FP32 = 0,      /* FP32 MFMA time DFT; every uint16 sample is exact */
   HFX_PREC_FP16 = 1,      /* FP16 MFMA, FP32 accumulate; samples exact up to 12 bits */
-  HFX_PREC_BF16 = 2,      /* BF16 MFMA, FP32 accumulate; samples rounded to 8 bits */
+  HFX_PREC_BF16 = 2,      /* BF16 MFMA, FP32 accumulate; samples rounded to 8 significant bits */
   HFX_PREC_FP16_SPLIT = 3 /* FP16 MFMA with hi/lo split twiddles, FP32 accumulate; samples
                              exact up to 12 bits */
 };
