@@ -551,12 +551,13 @@ def main():
             mp.close()
         mixed["fp32_rel_l2_vs_fp64"] = {k: float(f"{fp32_err[i]:.3e}") for i, k in enumerate(("M0", "M1", "M2"))}
         # The bound on any time-DFT speedup: the same batch with the time DFT (its frame loads and
-        # MFMAs) taken out of the fused front kernels, measured once with a timing-only build
-        # (rates of that run, same GPU and workload): FP32 pipelined front 8,959 -> 12,764, the
-        # 16-bit fused front 9,983 -> 14,572 moment frames/s.  A 16-bit DFT that cost nothing at
-        # all, frame loads included, would reach 14,572 / 8,959 = 1.63x the FP32 path.
-        mixed["ceiling_if_time_dft_free"] = {"fp32_batch_without_dft": 12764, "16bit_batch_without_dft": 14572,
-                                             "fp32_batch_same_run": 8959, "max_speedup_vs_fp32": 1.63,
+        # MFMAs) taken out of the fused front kernels, measured with a timing-only build of the
+        # current kernels (means of two runs, same GPU and workload): FP32 pipelined front
+        # 9,080 -> 12,724, the 16-bit fused fronts 10,015 -> 14,442 moment frames/s.  A 16-bit DFT
+        # that cost nothing at all, frame loads included, would reach 14,442 / 9,080 = 1.59x the
+        # FP32 path.
+        mixed["ceiling_if_time_dft_free"] = {"fp32_batch_without_dft": 12724, "16bit_batch_without_dft": 14442,
+                                             "fp32_batch_same_run": 9080, "max_speedup_vs_fp32": 1.59,
                                              "source": "design measurement (fused fronts built without the time DFT)"}
 
     del out
