@@ -252,6 +252,26 @@ __global__ __launch_bounds__(row_threads<W>()) void k_dft_row(DftArgs a) {
     asm volatile("v_mov_b32 %0, %1" : "=v"(lane) : "v"(tid & 63));
     f4 acc[TPW][NBT][4];
     dft_tile<NBT, MODE, TPW>(a.r, wmat, (long long)y * W + wave * 64 * TPW, lane, pre, acc);
+    // lens factors of the lane's pixels (1 without a lens: the product with (1, 0) is exact,
+    // and cheaper than a select)
+    auto lens_factors = [&](int i, float2 (&L)[4]) {
+      const int x0 = (wave * TPW + i) * 64 + 4 * (lane & 15);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) L[t] = make_float2(1.f, 0.f);
+      if (lens) {
+        const float2 ly = a.ly[y];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) L[t] = cmul(a.lx[x0 + t], ly);
+      }
+    };
+    // single group: the factors are loaded before the next row's prefetch (loaded after it,
+    // their wait would also wait for the prefetch: the vector-memory counter drains in order);
+    // MULTI loads them per group in the scatter, where registers are scarcer
+    float2 Ls[TPW][4];
+    if constexpr (!MULTI) {
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) lens_factors(i, Ls[i]);
+    }
     // the next row's first frames load while this row is scattered and transformed
     if (y + G < a.H) dft_prefetch<MODE, TPW>(a.r, (long long)(y + G) * W + wave * 64 * TPW, lane, pre);
 
@@ -260,13 +280,12 @@ __global__ __launch_bounds__(row_threads<W>()) void k_dft_row(DftArgs a) {
 #pragma unroll
       for (int i = 0; i < TPW; ++i) {
         const int x0 = (wave * TPW + i) * 64 + 4 * (lane & 15);  // first of the lane's 4 pixels
-        // lens factors; 1 without a lens (the product with (1, 0) is exact, and cheaper than a select)
-        float2 L[4] = {make_float2(1.f, 0.f), make_float2(1.f, 0.f), make_float2(1.f, 0.f), make_float2(1.f, 0.f)};
-        if (lens) {
-          const float2 ly = a.ly[y];
+        float2 L[4];
+        if constexpr (MULTI)
+          lens_factors(i, L);
+        else
 #pragma unroll
-          for (int t = 0; t < 4; ++t) L[t] = cmul(a.lx[x0 + t], ly);
-        }
+          for (int t = 0; t < 4; ++t) L[t] = Ls[i][t];
 #pragma unroll
         for (int bt = 0; bt < NBT; ++bt) {
 #pragma unroll
