@@ -442,7 +442,10 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
     const int y = row_y(i);
     const float2 *lxw = a.lx_w[win_of(i)], *lyw = a.ly_w[win_of(i)];
     // band bin `bin` of 4 pixels from x0 into the LDS plane rows (and its conjugate plane on
-    // the two-sided path), lens applied
+    // the two-sided path), lens applied.  The product's rounding is pinned (cmul_rn): the
+    // first row is scattered by the copy of this code before the row loop, the others by the
+    // copy inside it, and a window's first row may fall in either (a grouped launch), so both
+    // copies must round alike.
     auto put = [&](int x0, int bin, const float2 (&v)[4], const float2 (&L)[4]) {
 #pragma unroll
       for (int side = 0; side < 2; ++side) {
@@ -460,7 +463,7 @@ __global__ __launch_bounds__(front_threads<W>()) void k_front(DftArgs a) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           float2 c = side ? make_float2(v[t].x, -v[t].y) : v[t];
-          dst[F::pad(x0 + t)] = cmul(c, L[t]);
+          dst[F::pad(x0 + t)] = cmul_rn(c, L[t]);
         }
       }
     };
