@@ -145,6 +145,20 @@ def test_folded_front_vs_oracles(gpu, W, T, kb, ke, prop, both):
     _check(p, _frames(64, W, F), F - T // 4)
 
 
+@pytest.mark.parametrize("W", [512, 768, 1024])
+@pytest.mark.parametrize("T", [32, 64, 128])
+@pytest.mark.parametrize("kb,ke", [(3, 9), (2, 16)])
+def test_every_front_variant_vs_fp64(gpu, W, T, kb, ke):
+    """Each compiled FP32 front kernel (row widths 512/768/1024, windows 32/64/128, up to 8 bins
+    interleaved or 9..16 folded) against the FP64 oracle.  768 rows on at most 256 workgroups:
+    each workgroup runs three rows, so the row loop runs as well as its first and last row."""
+    H = 768
+    prop = "fresnel" if (W // 256 + T // 32) % 2 else "asm"
+    p = O.PathParams(H, W, T, T // 4, kb, ke, prop, 852e-9, 12e-6, 12e-6, 0.4, fftshift=True)
+    F = T + T // 4
+    _check(p, _frames(H, W, F, seed=W + T + kb), T // 4, against=("f64",))
+
+
 @pytest.mark.parametrize("H,W,T,kb,ke,prop,both", [(48, 64, 64, 1, 30, "asm", False),    # 29 bins: two tiles
                                                    (48, 64, 32, 0, 16, "fresnel", True),  # 17 bins with DC
                                                    (48, 64, 128, 10, 50, "fresnel", False),  # 40: three
