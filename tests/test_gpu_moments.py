@@ -131,6 +131,21 @@ def test_fused_16bit_front_vs_fp64(gpu, precision, tol, H, W, T, S, kb, ke, prop
         assert errs["f64"][0] <= 1e-4
 
 
+@pytest.mark.parametrize("precision,tol", [("fp16_split", 1e-5), ("fp16", 1e-3), ("bf16", 2e-2)])
+@pytest.mark.parametrize("W,kb,ke,both", [(512, 3, 9, False), (512, 2, 16, False), (768, 3, 9, False),
+                                          (768, 2, 16, False), (1024, 3, 9, False), (1024, 2, 16, False),
+                                          (768, 2, 16, True)])  # 30 planes: more than one LDS group
+def test_every_16bit_front_variant_vs_fp64(gpu, precision, tol, W, kb, ke, both):
+    """Each compiled 16-bit fused kernel (row widths 512/768/1024, one or two bin tiles, and the
+    several-group form of a two-sided band) against the FP64 oracle, with the bars of the modes."""
+    H, T = 256, 64
+    prop = "asm" if W == 768 else "fresnel"
+    p = O.PathParams(H, W, T, T // 4, kb, ke, prop, 852e-9, 12e-6, 12e-6, 0.4, fftshift=True,
+                     both_sidebands=both)
+    F = T + T // 4
+    _check(p, _frames(H, W, F, seed=W + kb), T // 4, precision, against=("f64",), tol=tol)
+
+
 @pytest.mark.parametrize("W,T,kb,ke,prop,both", [(512, 32, 0, 12, "fresnel", False),
                                                  (768, 64, 5, 17, "asm", False),
                                                  (512, 128, 20, 35, "fresnel", True),
