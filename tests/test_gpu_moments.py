@@ -205,6 +205,19 @@ def test_full_16bit_samples_fp32(gpu, H, W, kb, ke, prop):
     _check(p, ring, 3)
 
 
+@pytest.mark.parametrize("precision,tol", [("fp32", TOL_FP32), ("fp16_split", 1e-5), ("fp16", 1e-3),
+                                           ("bf16", 2e-2)])
+@pytest.mark.parametrize("kb,ke", [(1, 7), (1, 15), (1, 31), (1, 47), (1, 65)])
+def test_every_standalone_dft_variant_vs_fp64(gpu, precision, tol, kb, ke):
+    """The stand-alone time DFT (a width without the fused kernel) for 6, 14, 30, 46 and 64
+    bins at T = 128: in FP32 the interleaved kernel and the folded one over one to four 16-bin
+    tiles, in the 16-bit modes the interleaved kernel over 1, 2, 4, 6 and 8 row tiles."""
+    H, W, T = 48, 64, 128
+    p = O.PathParams(H, W, T, T // 4, kb, ke, "fresnel", 852e-9, 12e-6, 12e-6, 0.4, fftshift=True)
+    F = T + T // 4
+    _check(p, _frames(H, W, F, seed=ke), T // 4, precision, against=("f64",), tol=tol)
+
+
 def test_fresnel_band_split_1024_rows(gpu):
     """Band-split moments (five sums) out of the 1024-row column pass, against both oracles."""
     p = O.PathParams(1024, 256, 32, 8, 4, 12, "fresnel", 852e-9, 12e-6, 12e-6, 0.45, fftshift=True,
