@@ -218,6 +218,19 @@ def test_every_standalone_dft_variant_vs_fp64(gpu, precision, tol, kb, ke):
     _check(p, _frames(H, W, F, seed=ke), T // 4, precision, against=("f64",), tol=tol)
 
 
+@pytest.mark.parametrize("prop", ["fresnel", "asm"])
+@pytest.mark.parametrize("H,W", [(256, 256), (512, 768), (768, 512), (1024, 256), (256, 1024), (2048, 256),
+                                 (256, 2048)])
+def test_every_pass_length_vs_fp64(gpu, prop, H, W):
+    """The wave-level row and column passes at every length the path compiles (256, 512, 768,
+    1024, 2048 points, rows and columns), through the fused front or the stand-alone time DFT
+    with the forward row pass, for both propagations, against the FP64 oracle."""
+    T = 32
+    p = O.PathParams(H, W, T, T // 4, 2, 9, prop, 852e-9, 12e-6, 12e-6, 0.4, fftshift=True)
+    F = T + T // 4
+    _check(p, _frames(H, W, F, seed=H + W), T // 4, against=("f64",))
+
+
 def test_fresnel_band_split_1024_rows(gpu):
     """Band-split moments (five sums) out of the 1024-row column pass, against both oracles."""
     p = O.PathParams(1024, 256, 32, 8, 4, 12, "fresnel", 852e-9, 12e-6, 12e-6, 0.45, fftshift=True,
